@@ -1,0 +1,101 @@
+/*
+ * nnh.h — C ABI of the MI355X three-prime NTT multiply (libnnh.so).
+ *
+ * Limbs are little-endian uint64_t words, least significant first, exactly
+ * the layout of the reference's basic_natural (basic_natural.h:11).  Every
+ * entry point returns an nnh_status; nothing throws across the boundary.
+ * Buffers are caller-owned; the library keeps per-device twiddle tables and
+ * a growing workspace.  Calls on one device are serialised by an internal
+ * lock; each call completes on the given stream before it returns unless the
+ * name ends in _async.
+ *
+ * The entry points replace the reference's multiply choice points:
+ *   natural::_mul / _mul_NTT     natural.h:391-450   -> nnh_mul
+ *   natural::_sqr / _sqr_NTT     natural.h:433-442, 529-535 -> nnh_sqr
+ *   NTT_info::load+NTT           ntt_info.h:147-198  -> nnh_transform_forward
+ *   mul_load_NTT_info (reuse)    natural.h:504-527   -> nnh_mul_transformed
+ *   NTT_info::save (cyclic mode) ntt_info.h:237-248  -> nnh_mul_cyclic
+ * and, with no counterpart in the reference (which multiplies one pair at
+ * a time), nnh_mul_batch for many independent products.
+ */
+#ifndef NNH_H
+#define NNH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nnh_status {
+    NNH_OK = 0,
+    NNH_ERR_ARG = 1,       /* null pointer, zero length, size out of range */
+    NNH_ERR_NOMEM = 2,     /* device allocation failed */
+    NNH_ERR_HIP = 3,       /* a HIP runtime call failed */
+    NNH_ERR_NODEV = 4      /* no usable device / nnh_init not called */
+} nnh_status;
+
+/* Select and initialise a device (builds the root-base constants).
+ * Reference: the static tables of NTT_info (ntt_info.h:69-145). */
+int nnh_init(int device);
+
+/* Release every device buffer held by the library. */
+int nnh_shutdown(void);
+
+/* dst[0 .. an+bn) = a * b for host buffers.  dst is not normalised: the
+ * caller trims top zero words (natural::std, natural.h:82-93).
+ * Reference: natural::_mul (natural.h:444-450). */
+int nnh_mul(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn);
+
+/* dst[0 .. 2an) = a^2.  Reference: natural::_sqr (natural.h:529-535). */
+int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an);
+
+/* Device-pointer variants: operands and result already in HBM, work is
+ * queued on `stream` (a hipStream_t, NULL = the library's stream) and the
+ * call returns without synchronising. */
+int nnh_mul_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an,
+                         const uint64_t* d_b, uint64_t bn, void* stream);
+int nnh_sqr_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, void* stream);
+
+/* Cyclic product (a*b) mod (2^(64*2^scale) - 1) as the reference's
+ * NTT_info::save computes it when the product is longer than the transform
+ * (ntt_info.h:238-242): dst receives 2^scale words.  Requires
+ * an, bn <= 2^scale.  Host buffers. */
+int nnh_mul_cyclic(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn, int scale);
+
+/* Forward transform of a at `scale` (NTT_info::load + NTT_info::NTT,
+ * ntt_info.h:147-198) into out[3 * 2^scale] (prime-major).  The words are
+ * the reference's transform multiplied by alpha_scale (see nnh_alpha) and
+ * may be lazily reduced into [0, 2p).  Host buffers. */
+int nnh_transform_forward(uint64_t* out, const uint64_t* a, uint64_t an, int scale);
+
+/* The constant alpha_scale mod p_i (i = 0..2) by which device transforms
+ * differ from the reference's. */
+int nnh_alpha(int scale, uint64_t out[3]);
+
+/* dst = a * b where fa is a forward transform of a at `scale` produced by
+ * nnh_transform_forward (mul_load_NTT_info, natural.h:504-515).  Requires
+ * an + bn - 1 <= 2^scale.  Host buffers. */
+int nnh_mul_transformed(uint64_t* dst, const uint64_t* fa, uint64_t an, const uint64_t* b, uint64_t bn, int scale);
+
+/* Smallest operand size (words) at which nnh_mul uses the transform
+ * (the reference's NTTmul_threshold, ntt_info.h:6). */
+uint64_t nnh_threshold_words(void);
+
+/* Kernel timing (no counterpart in the reference; for benchmarks): with
+ * timing enabled the library records HIP events on the launch stream around
+ * each kernel family; nnh_kernel_times reads the accumulated milliseconds,
+ * launch count, algorithmic HBM bytes and name of family `kind`
+ * (0 .. NNH_KERNEL_KINDS-1).  Enabling or disabling clears the totals. */
+#define NNH_KERNEL_KINDS 7
+int nnh_kernel_timing(int enable);
+int nnh_kernel_times(int kind, double* total_ms, uint64_t* launches, double* bytes, const char** name);
+
+/* Human-readable text of the last error on this thread. */
+const char* nnh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NNH_H */

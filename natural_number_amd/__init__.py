@@ -1,0 +1,180 @@
+"""natural_number_amd — MI355X three-prime NTT multiply of large naturals.
+
+The compute lives in libnnh.so (HIP kernels for gfx950 behind the C ABI in
+include/nnh.h).  This module is a thin ctypes binding used by the tests and
+the benchmark; it never computes a product itself and raises NNHError when
+the library is missing or a call fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "NNHError", "lib", "mul", "sqr", "mul_cyclic", "transform_forward", "alpha",
+    "mul_transformed", "mul_device_async", "sqr_device_async", "threshold_words", "kernel_timing", "kernel_times",
+    "to_limbs", "from_limbs", "splitmix64",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnnh.so")
+_lib = None
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class NNHError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libnnh.so (built in-tree by natural_number_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NNHError(f"{LIB_PATH} is not built; run `python -m natural_number_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    u64, vp, i32 = ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int
+    sigs = {
+        "nnh_init": ([i32], i32),
+        "nnh_shutdown": ([], i32),
+        "nnh_mul": ([_u64p, _u64p, u64, _u64p, u64], i32),
+        "nnh_sqr": ([_u64p, _u64p, u64], i32),
+        "nnh_mul_device_async": ([vp, vp, u64, vp, u64, vp], i32),
+        "nnh_sqr_device_async": ([vp, vp, u64, vp], i32),
+        "nnh_mul_cyclic": ([_u64p, _u64p, u64, _u64p, u64, i32], i32),
+        "nnh_transform_forward": ([_u64p, _u64p, u64, i32], i32),
+        "nnh_alpha": ([i32, _u64p], i32),
+        "nnh_mul_transformed": ([_u64p, _u64p, u64, _u64p, u64, i32], i32),
+        "nnh_threshold_words": ([], u64),
+        "nnh_kernel_timing": ([i32], i32),
+        "nnh_kernel_times": ([i32, ctypes.POINTER(ctypes.c_double), _u64p, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_char_p)], i32),
+        "nnh_last_error": ([], ctypes.c_char_p),
+    }
+    for name, (args, res) in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise NNHError(f"nnh status {rc}: {lib().nnh_last_error().decode(errors='replace')}")
+
+
+def _arr(x):
+    return x.ctypes.data_as(_u64p)
+
+
+def _limbs(x):
+    a = np.ascontiguousarray(x, dtype=np.uint64)
+    if a.ndim != 1 or a.size == 0:
+        raise ValueError("operands are non-empty 1-D uint64 limb arrays")
+    return a
+
+
+def mul(a, b):
+    """a * b as an+bn little-endian words (not normalised)."""
+    a, b = _limbs(a), _limbs(b)
+    out = np.empty(a.size + b.size, dtype=np.uint64)
+    _check(lib().nnh_mul(_arr(out), _arr(a), a.size, _arr(b), b.size))
+    return out
+
+
+def sqr(a):
+    a = _limbs(a)
+    out = np.empty(2 * a.size, dtype=np.uint64)
+    _check(lib().nnh_sqr(_arr(out), _arr(a), a.size))
+    return out
+
+
+def mul_cyclic(a, b, scale):
+    """Product through one transform of 2^scale words; mod 2^(64*2^scale)-1
+    (2^scale words) when longer than the transform, else exact."""
+    a, b = _limbs(a), _limbs(b)
+    L = 1 << scale
+    n = L if a.size + b.size - 1 > L else a.size + b.size
+    out = np.empty(max(n, a.size + b.size), dtype=np.uint64)
+    _check(lib().nnh_mul_cyclic(_arr(out), _arr(a), a.size, _arr(b), b.size, scale))
+    return out[:n]
+
+
+def transform_forward(a, scale):
+    a = _limbs(a)
+    out = np.empty(3 << scale, dtype=np.uint64)
+    _check(lib().nnh_transform_forward(_arr(out), _arr(a), a.size, scale))
+    return out
+
+
+def alpha(scale):
+    out = np.empty(3, dtype=np.uint64)
+    _check(lib().nnh_alpha(scale, _arr(out)))
+    return [int(x) for x in out]
+
+
+def mul_transformed(fa, an, b, scale):
+    fa, b = _limbs(fa), _limbs(b)
+    out = np.empty(an + b.size, dtype=np.uint64)
+    _check(lib().nnh_mul_transformed(_arr(out), _arr(fa), an, _arr(b), b.size, scale))
+    return out
+
+
+def mul_device_async(d_out, d_a, an, d_b, bn, stream=None):
+    """Device-pointer product (pointers as ints), queued on `stream`."""
+    _check(lib().nnh_mul_device_async(d_out, d_a, an, d_b, bn, stream))
+
+
+def sqr_device_async(d_out, d_a, an, stream=None):
+    _check(lib().nnh_sqr_device_async(d_out, d_a, an, stream))
+
+
+def kernel_timing(enable):
+    _check(lib().nnh_kernel_timing(1 if enable else 0))
+
+
+def kernel_times():
+    """{name: (total_ms, launches, algorithmic_bytes)} since timing was enabled."""
+    res = {}
+    for k in range(7):
+        ms, n, by, nm = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double(), ctypes.c_char_p()
+        _check(lib().nnh_kernel_times(k, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(by), ctypes.byref(nm)))
+        res[nm.value.decode()] = (ms.value, int(n.value), by.value)
+    return res
+
+
+def threshold_words():
+    return int(lib().nnh_threshold_words())
+
+
+# ---- host-side helpers (no arithmetic on the product path) -----------------
+def to_limbs(x, n=None):
+    if n is None:
+        n = max(1, (x.bit_length() + 63) // 64)
+    return np.frombuffer(x.to_bytes(8 * n, "little"), dtype=np.uint64).copy()
+
+
+def from_limbs(w):
+    return int.from_bytes(np.ascontiguousarray(w, dtype=np.uint64).tobytes(), "little")
+
+
+_G = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed, n, top_nonzero=True):
+    """n words of Vigna's splitmix64 stream from `seed` (state += gamma, then
+    the two xor-shift-multiply rounds); the top word is forced nonzero."""
+    with np.errstate(over="ignore"):
+        i = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * _G
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    if top_nonzero and n and z[-1] == 0:
+        z[-1] = 1
+    return z

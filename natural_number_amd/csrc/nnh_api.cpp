@@ -1,0 +1,666 @@
+// nnh_api.cpp — host driver and C ABI of libnnh.so.
+//
+// Owns the per-device state (root-base constants, Montgomery-form twiddle
+// tables grown on demand like the reference's static NTT_info::root/iroot
+// caches, ntt_info.h:22-41 and 141-145, and a growing workspace) and
+// sequences the kernels of one product:
+//   top passes (limbs -> residues, transform levels h >= 4096)
+//   -> fused chunk kernel (remaining levels of A and B, base multiply,
+//      inverse levels h < 4096)
+//   -> inverse top passes -> Garner CRT + carry scan.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <bit>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nnh.h"
+#include "nnh_internal.hpp"
+
+namespace nnh {
+
+typedef unsigned __int128 u128;
+
+constexpr u64 P[3] = {27ull << 56 | 1, 58ull << 56 | 1, 87ull << 56 | 1};
+// Primitive 2^56-th roots of unity: the last entry of each row of the
+// reference's root_base table (ntt_info.h:89, 98, 107); every other entry of
+// that table is a repeated square of it.
+constexpr u64 ROOT_GEN[3] = {11, 2, 310};
+constexpr int MIN_MID_BITS = 8;       // smallest fused-chunk transform (2^8 words)
+constexpr int TILE_BITS = 12;         // fused chunk / top-pass tile: 4096 words
+constexpr int MAX_SCALE = 30;         // largest transform handled (2^30 words per prime)
+constexpr u64 NTT_MUL_THRESHOLD = 384;  // reference NTTmul_threshold (ntt_info.h:6)
+
+static u64 mulmod(u64 a, u64 b, u64 p) { return (u64)((u128)a * b % p); }
+static u64 powmod(u64 b, u64 e, u64 p) {
+    u64 r = 1;
+    for (b %= p; e; e >>= 1, b = mulmod(b, b, p))
+        if (e & 1) r = mulmod(r, b, p);
+    return r;
+}
+static u64 to_mont(u64 x, u64 p) { return (u64)(((u128)(x % p) << 64) % p); }
+static int ntt_scale(u64 s1, u64 s2) { return 64 - std::countl_zero(s1 + s2 - 2); }
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess)                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? NNH_ERR_NOMEM : NNH_ERR_HIP,   \
+                        std::string(#x) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+struct Buf {
+    u64* p = nullptr;
+    u64 words = 0;
+};
+
+struct Device {
+    int id = -1;
+    hipStream_t stream = nullptr;
+    RootBases rb{};
+    CrtConsts crt{};
+    u64 rb_plain[3][56]{}, irb_plain[3][56]{};
+    int tw_scale = 0;          // tables hold 2^(tw_scale-2) entries
+    u64* twf[3]{};
+    u64* twi[3]{};
+    Buf ws;                    // 6 L words: A and B transforms
+    Buf tiles;                 // tile carry functions and carry-ins (bytes)
+    Buf stage;                 // host-API staging: a, b, dst
+    Buf tmp;                   // split addend / cyclic words / columns
+    std::mutex mu;
+};
+
+static Device g_dev;
+static bool g_inited = false;
+static std::mutex g_init_mu;
+
+// ---------------------------------------------------------------------------
+// Optional per-kernel timing: HIP events recorded on the launch stream around
+// each kernel family, summed when read (nnh_kernel_times).
+// ---------------------------------------------------------------------------
+enum KernelKind { K_TOP_LOAD = 0, K_TOP_FWD, K_MID, K_TOP_INV, K_CRT, K_SCHOOL, K_FOLD, K_NKINDS };
+static const char* KIND_NAMES[K_NKINDS] = {"top_pass_load", "top_pass_fwd", "mid_fused", "top_pass_inv",
+                                           "crt_carry", "school_cols", "cyclic_fold"};
+struct ProfRec {
+    hipEvent_t a, b;
+    int kind;
+    u64 bytes;
+};
+static bool g_prof = false;
+static std::vector<ProfRec> g_prof_recs;
+static double g_prof_ms[K_NKINDS];
+static u64 g_prof_n[K_NKINDS];
+static double g_prof_bytes[K_NKINDS];
+struct ProfScope {
+    ProfRec r{};
+    hipStream_t st;
+    bool on;
+    ProfScope(int kind, u64 bytes, hipStream_t s) : st(s), on(g_prof) {
+        if (!on) return;
+        r.kind = kind;
+        r.bytes = bytes;
+        (void)hipEventCreate(&r.a);
+        (void)hipEventCreate(&r.b);
+        (void)hipEventRecord(r.a, st);
+    }
+    ~ProfScope() {
+        if (!on) return;
+        (void)hipEventRecord(r.b, st);
+        g_prof_recs.push_back(r);
+    }
+};
+static void prof_collect() {
+    for (ProfRec& r : g_prof_recs) {
+        float ms = 0;
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        g_prof_ms[r.kind] += ms;
+        g_prof_n[r.kind] += 1;
+        g_prof_bytes[r.kind] += (double)r.bytes;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof_recs.clear();
+}
+
+static int ensure(Buf& b, u64 words) {
+    if (b.words >= words) return NNH_OK;
+    // queued work on any stream may still use the old buffer
+    if (b.p) HIPCHK(hipDeviceSynchronize());
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.words = 0;
+    u64 w = std::bit_ceil(std::max<u64>(words, 1024));
+    HIPCHK(hipMalloc((void**)&b.p, w * sizeof(u64)));
+    b.words = w;
+    return NNH_OK;
+}
+
+// alpha_s with alpha^2 = 2^64 * 2^-(s-1) (mod p): scales every loaded
+// residue so that the base multiply's Montgomery factor 2^-64 and the
+// unscaled inverse's 2^(s-1) cancel.  sqrt(2) = z + z^-1 for a primitive
+// 8th root z = rb[2].
+static u64 alpha_plain(const Device& D, int i, int s) {
+    u64 p = P[i];
+    int e = 65 - s;
+    u64 a = powmod(2, (u64)(e / 2), p);
+    if (e & 1) {
+        u64 sqrt2 = (D.rb_plain[i][2] + D.irb_plain[i][2]) % p;
+        a = mulmod(a, sqrt2, p);
+    }
+    return a;
+}
+
+static void build_constants(Device& D) {
+    for (int i = 0; i < 3; i++) {
+        u64 p = P[i];
+        D.rb_plain[i][55] = ROOT_GEN[i];
+        for (int j = 54; j >= 0; j--) D.rb_plain[i][j] = mulmod(D.rb_plain[i][j + 1], D.rb_plain[i][j + 1], p);
+        for (int j = 0; j < 56; j++) {
+            D.irb_plain[i][j] = powmod(D.rb_plain[i][j], p - 2, p);
+            D.rb.rb_m[i][j] = to_mont(D.rb_plain[i][j], p);
+            D.rb.irb_m[i][j] = to_mont(D.irb_plain[i][j], p);
+        }
+        D.rb.one_m[i] = to_mont(1, p);
+    }
+    u64 m0 = P[0], m1 = P[1], m2 = P[2];
+    D.crt.c02m = to_mont(powmod(m0, m2 - 2, m2), m2);
+    D.crt.m0m = to_mont(m0 % m1, m1);
+    D.crt.c1m = to_mont(powmod(mulmod(m0 % m1, m2 % m1, m1), m1 - 2, m1), m1);
+    u128 m02 = (u128)m0 * m2;
+    D.crt.m02lo = (u64)m02;
+    D.crt.m02hi = (u64)(m02 >> 64);
+}
+
+static int ensure_twiddles(Device& D, int s) {
+    if (s <= D.tw_scale) return NNH_OK;
+    int ns = std::max(s, 12);
+    u64 n = 1ull << (ns - 2);
+    if (D.tw_scale) HIPCHK(hipDeviceSynchronize());
+    for (int i = 0; i < 3; i++) {
+        if (D.twf[i]) (void)hipFree(D.twf[i]);
+        if (D.twi[i]) (void)hipFree(D.twi[i]);
+        D.twf[i] = D.twi[i] = nullptr;
+    }
+    D.tw_scale = 0;
+    for (int i = 0; i < 3; i++) {
+        HIPCHK(hipMalloc((void**)&D.twf[i], n * sizeof(u64)));
+        HIPCHK(hipMalloc((void**)&D.twi[i], n * sizeof(u64)));
+        launch_twiddles(D.twf[i], n, D.rb, i, 0, D.stream);
+        launch_twiddles(D.twi[i], n, D.rb, i, 1, D.stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(D.stream));
+    D.tw_scale = ns;
+    return NNH_OK;
+}
+
+static int device_init(int dev) {
+    std::lock_guard<std::mutex> g(g_init_mu);
+    if (g_inited && g_dev.id == dev) return NNH_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= dev || dev < 0)
+        return fail(NNH_ERR_NODEV, "no HIP device " + std::to_string(dev));
+    HIPCHK(hipSetDevice(dev));
+    if (!g_dev.stream) HIPCHK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    g_dev.id = dev;
+    build_constants(g_dev);
+    g_inited = true;
+    return NNH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Transform building blocks (device pointers, queued on st)
+// ---------------------------------------------------------------------------
+struct Plan {
+    int s;
+    u64 L;
+    int cb;          // fused chunk bits (min(s, 12))
+    int ntop;        // top levels (s - cb)
+    u64 alpha_m[3];
+};
+
+static Plan make_plan(const Device& D, int s) {
+    Plan pl;
+    pl.s = s;
+    pl.L = 1ull << s;
+    pl.cb = std::min(s, TILE_BITS);
+    pl.ntop = s - pl.cb;
+    for (int i = 0; i < 3; i++) pl.alpha_m[i] = to_mont(alpha_plain(D, i, s), P[i]);
+    return pl;
+}
+
+static void set3(u64* dst[3], u64* base, u64 L) {
+    for (int i = 0; i < 3; i++) dst[i] = base + (u64)i * L;
+}
+
+// Forward top levels (global bits s-1 .. 12) of operand x into f.
+static void fwd_top(const Device& D, const Plan& pl, const u64* x, u64 xn, u64* f, hipStream_t st) {
+    int g_hi = pl.s - 1;
+    bool first = true;
+    int remaining = pl.ntop;
+    while (remaining > 0) {
+        int nl = std::min(remaining, 8);
+        TopPassDesc td{};
+        set3(td.data, f, pl.L);
+        for (int i = 0; i < 3; i++) {
+            td.tw[i] = D.twf[i];
+            td.alpha_m[i] = pl.alpha_m[i];
+        }
+        td.limbs = first ? x : nullptr;
+        td.an = xn;
+        td.L = pl.L;
+        td.g_lo = (u32)(g_hi - nl + 1);
+        td.nlev = (u32)nl;
+        td.inverse = false;
+        {
+            // algorithmic bytes: limbs read once (first pass) or residues read, residues written
+            u64 rd = first ? 8 * std::min<u64>(xn, pl.L) : 24 * pl.L;
+            ProfScope ps(first ? K_TOP_LOAD : K_TOP_FWD, rd + 24 * pl.L, st);
+            launch_top_pass(td, st);
+        }
+        g_hi -= nl;
+        remaining -= nl;
+        first = false;
+    }
+}
+
+// Inverse top levels (global bits 12 .. s-1), mirrored passes.
+static void inv_top(const Device& D, const Plan& pl, u64* f, hipStream_t st) {
+    std::vector<std::pair<int, int>> passes;  // (g_lo, nlev) in forward order
+    int g_hi = pl.s - 1, remaining = pl.ntop;
+    while (remaining > 0) {
+        int nl = std::min(remaining, 8);
+        passes.push_back({g_hi - nl + 1, nl});
+        g_hi -= nl;
+        remaining -= nl;
+    }
+    for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
+        TopPassDesc td{};
+        set3(td.data, f, pl.L);
+        for (int i = 0; i < 3; i++) {
+            td.tw[i] = D.twi[i];
+            td.alpha_m[i] = pl.alpha_m[i];
+        }
+        td.limbs = nullptr;
+        td.L = pl.L;
+        td.g_lo = (u32)it->first;
+        td.nlev = (u32)it->second;
+        td.inverse = true;
+        {
+            ProfScope ps(K_TOP_INV, 48 * pl.L, st);
+            launch_top_pass(td, st);
+        }
+    }
+}
+
+// Fused chunk kernel.  a/b are limbs when the transform fits one chunk and
+// the operand is not yet transformed.
+static void mid(const Device& D, const Plan& pl, u64* fa, const u64* fb, const u64* la, u64 an,
+                const u64* lb, u64 bn, bool sqr, bool fwd_only, bool a_done, bool b_done, hipStream_t st) {
+    MidDesc md{};
+    set3(md.fa, fa, pl.L);
+    for (int i = 0; i < 3; i++) {
+        md.fb[i] = fb + (u64)i * pl.L;
+        md.twf[i] = D.twf[i];
+        md.twi[i] = D.twi[i];
+        md.alpha_m[i] = pl.alpha_m[i];
+    }
+    md.la = la;
+    md.lb = lb;
+    md.an = an;
+    md.bn = bn;
+    md.L = pl.L;
+    md.cb = pl.cb;
+    md.sqr = sqr;
+    md.fwd_only = fwd_only;
+    md.a_done = a_done;
+    md.b_done = b_done;
+    // bytes: operands read (limbs or residues), result written
+    u64 ra = la ? 8 * std::min<u64>(an, pl.L) : 24 * pl.L;
+    u64 rb = fwd_only || sqr ? 0 : (lb ? 8 * std::min<u64>(bn, pl.L) : 24 * pl.L);
+    ProfScope ps(K_MID, ra + rb + 24 * pl.L, st);
+    launch_mid(md, st);
+}
+
+static int crt(Device& D, const Plan& pl, const u64* f, u64 n_res, u64* out, u64 n_out, const u64* addend,
+               u64 n_add, hipStream_t st) {
+    u64 tiles = crt_tiles(n_out);
+    int rc = ensure(D.tiles, (2 * tiles + 7) / 8 + 16);
+    if (rc) return rc;
+    CrtDesc cd{};
+    for (int i = 0; i < 3; i++) cd.r[i] = f + (u64)i * pl.L;
+    cd.columns = 0;
+    cd.n_res = n_res;
+    cd.out = out;
+    cd.n_out = n_out;
+    cd.addend = addend;
+    cd.n_add = n_add;
+    cd.tile_fn = (u8*)D.tiles.p;
+    cd.tile_cin = (u8*)D.tiles.p + tiles;
+    cd.C = D.crt;
+    {
+        ProfScope ps(K_CRT, 24 * n_res + 8 * n_out + 8 * n_add, st);
+        launch_crt(cd, st);
+    }
+    return NNH_OK;
+}
+
+// Exact product of transforms already prepared: fa holds A (limbs loaded by
+// the caller's top passes, or a finished transform if a_done), similarly fb.
+// Writes out[0 .. n_out) (+ addend).
+static int product_core(Device& D, const Plan& pl, const u64* a, u64 an, const u64* b, u64 bn, bool sqr,
+                        const u64* fb_done, u64* out, u64 n_out, const u64* addend, u64 n_add, hipStream_t st) {
+    u64* fa = D.ws.p;
+    u64* fb = D.ws.p + 3 * pl.L;
+    const u64* fbp = fb_done ? fb_done : fb;
+    bool small = pl.ntop == 0;
+    if (!small) {
+        fwd_top(D, pl, a, an, fa, st);
+        if (!sqr && !fb_done) fwd_top(D, pl, b, bn, fb, st);
+    }
+    mid(D, pl, fa, fbp, small ? a : nullptr, an, (small && !sqr && !fb_done) ? b : nullptr, bn, sqr, false,
+        false, fb_done != nullptr, st);
+    if (!small) inv_top(D, pl, fa, st);
+    u64 n_res = std::min<u64>(an + (sqr ? an : bn) - 1, pl.L);
+    return crt(D, pl, fa, n_res, out, n_out, addend, n_add, st);
+}
+
+// Complete forward transform of x into dst (3 L words).
+static void transform_full(Device& D, const Plan& pl, const u64* x, u64 xn, u64* dst, hipStream_t st) {
+    bool small = pl.ntop == 0;
+    if (!small) fwd_top(D, pl, x, xn, dst, st);
+    mid(D, pl, dst, dst, small ? x : nullptr, xn, nullptr, 0, false, true, false, false, st);
+}
+
+static int school(Device& D, const u64* a, u64 an, const u64* b, u64 bn, u64* out, hipStream_t st) {
+    u64 n = an + bn - 1;
+    int rc = ensure(D.tmp, 3 * n + 64);
+    if (rc) return rc;
+    u64* c0 = D.tmp.p;
+    u64* c1 = c0 + n;
+    u64* c2 = c1 + n;
+    {
+        ProfScope ps(K_SCHOOL, 8 * (an + bn) + 24 * n, st);
+        launch_school_cols(a, an, b, bn, c0, c1, c2, st);
+    }
+    u64 n_out = an + bn;
+    u64 tiles = crt_tiles(n_out);
+    rc = ensure(D.tiles, (2 * tiles + 7) / 8 + 16);
+    if (rc) return rc;
+    CrtDesc cd{};
+    cd.r[0] = c0; cd.r[1] = c1; cd.r[2] = c2;
+    cd.columns = 1;
+    cd.n_res = n;
+    cd.out = out;
+    cd.n_out = n_out;
+    cd.tile_fn = (u8*)D.tiles.p;
+    cd.tile_cin = (u8*)D.tiles.p + tiles;
+    cd.C = D.crt;
+    {
+        ProfScope ps(K_CRT, 24 * n + 8 * n_out, st);
+        launch_crt(cd, st);
+    }
+    return NNH_OK;
+}
+
+// out[0 .. an+bn) = a * b (device pointers), reference natural::_mul:
+// small operands by schoolbook columns, otherwise the transform with the
+// reference's one-level split of an unbalanced larger operand
+// (natural.h:391-431): B is transformed once and reused for both halves.
+static int mul_device(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st) {
+    if (an < bn) {
+        std::swap(a, b);
+        std::swap(an, bn);
+    }
+    if (bn < NTT_MUL_THRESHOLD) return school(D, a, an, b, bn, out, st);
+    u64 a0n = (an + 1) >> 1;
+    int s = ntt_scale(an, bn), s_opt = ntt_scale(a0n, bn);
+    if (s > MAX_SCALE) return fail(NNH_ERR_ARG, "operands too large");
+    int rc;
+    if (s == s_opt) {
+        Plan pl = make_plan(D, s);
+        if ((rc = ensure_twiddles(D, s)) || (rc = ensure(D.ws, 6 * pl.L))) return rc;
+        return product_core(D, pl, a, an, b, bn, false, nullptr, out, an + bn, nullptr, 0, st);
+    }
+    Plan pl = make_plan(D, s_opt);
+    if ((rc = ensure_twiddles(D, s_opt)) || (rc = ensure(D.ws, 6 * pl.L)) || (rc = ensure(D.tmp, bn + 64)))
+        return rc;
+    u64* fb = D.ws.p + 3 * pl.L;
+    transform_full(D, pl, b, bn, fb, st);
+    // low half: a0 * b -> out[0 .. a0n + bn)
+    if ((rc = product_core(D, pl, a, a0n, b, bn, false, fb, out, a0n + bn, nullptr, 0, st))) return rc;
+    // high half: a1 * b + (overlap) -> out[a0n .. an + bn)
+    HIPCHK(hipMemcpyAsync(D.tmp.p, out + a0n, bn * sizeof(u64), hipMemcpyDeviceToDevice, st));
+    return product_core(D, pl, a + a0n, an - a0n, b, bn, false, fb, out + a0n, an - a0n + bn, D.tmp.p, bn, st);
+}
+
+static int sqr_device(Device& D, u64* out, const u64* a, u64 an, hipStream_t st) {
+    if (an < NTT_MUL_THRESHOLD) return school(D, a, an, a, an, out, st);
+    int s = ntt_scale(an, an);
+    if (s > MAX_SCALE) return fail(NNH_ERR_ARG, "operand too large");
+    Plan pl = make_plan(D, s);
+    int rc;
+    if ((rc = ensure_twiddles(D, s)) || (rc = ensure(D.ws, 6 * pl.L))) return rc;
+    return product_core(D, pl, a, an, a, an, true, nullptr, out, 2 * an, nullptr, 0, st);
+}
+
+static int stage_host(Device& D, u64 words) { return ensure(D.stage, words); }
+
+}  // namespace nnh
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace nnh;
+
+#define NEED_INIT()                                                              \
+    do {                                                                         \
+        if (!g_inited) {                                                         \
+            int rc_ = device_init(0);                                            \
+            if (rc_) return rc_;                                                 \
+        }                                                                        \
+    } while (0)
+
+extern "C" {
+
+int nnh_init(int device) { return device_init(device); }
+
+int nnh_shutdown(void) {
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    if (!g_inited) return NNH_OK;
+    for (int i = 0; i < 3; i++) {
+        if (g_dev.twf[i]) (void)hipFree(g_dev.twf[i]);
+        if (g_dev.twi[i]) (void)hipFree(g_dev.twi[i]);
+        g_dev.twf[i] = g_dev.twi[i] = nullptr;
+    }
+    g_dev.tw_scale = 0;
+    for (Buf* b : {&g_dev.ws, &g_dev.tiles, &g_dev.stage, &g_dev.tmp}) {
+        if (b->p) (void)hipFree(b->p);
+        *b = Buf{};
+    }
+    return NNH_OK;
+}
+
+uint64_t nnh_threshold_words(void) { return NTT_MUL_THRESHOLD; }
+
+int nnh_kernel_timing(int enable) {
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    prof_collect();
+    g_prof = enable != 0;
+    for (int k = 0; k < K_NKINDS; k++) {
+        g_prof_ms[k] = 0;
+        g_prof_n[k] = 0;
+        g_prof_bytes[k] = 0;
+    }
+    return NNH_OK;
+}
+
+int nnh_kernel_times(int kind, double* total_ms, uint64_t* launches, double* bytes, const char** name) {
+    if (kind < 0 || kind >= K_NKINDS) return fail(NNH_ERR_ARG, "bad kernel kind");
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    prof_collect();
+    if (total_ms) *total_ms = g_prof_ms[kind];
+    if (launches) *launches = g_prof_n[kind];
+    if (bytes) *bytes = g_prof_bytes[kind];
+    if (name) *name = KIND_NAMES[kind];
+    return NNH_OK;
+}
+
+const char* nnh_last_error(void) { return g_err.c_str(); }
+
+int nnh_mul_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
+                         void* stream) {
+    if (!d_dst || !d_a || !d_b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
+    int rc = mul_device(g_dev, d_dst, d_a, an, d_b, bn, st);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    return NNH_OK;
+}
+
+int nnh_sqr_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, void* stream) {
+    if (!d_dst || !d_a || !an) return fail(NNH_ERR_ARG, "null pointer or empty operand");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
+    int rc = sqr_device(g_dev, d_dst, d_a, an, st);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    return NNH_OK;
+}
+
+int nnh_mul(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn) {
+    if (!dst || !a || !b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    Device& D = g_dev;
+    int rc = stage_host(D, 2 * (an + bn) + 16);
+    if (rc) return rc;
+    u64* da = D.stage.p;
+    u64* db = da + an;
+    u64* dd = db + bn;
+    hipStream_t st = D.stream;
+    HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
+    if ((rc = mul_device(D, dd, da, an, db, bn, st))) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst, dd, (an + bn) * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NNH_OK;
+}
+
+int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an) {
+    if (!dst || !a || !an) return fail(NNH_ERR_ARG, "null pointer or empty operand");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    Device& D = g_dev;
+    int rc = stage_host(D, 3 * an + 16);
+    if (rc) return rc;
+    u64* da = D.stage.p;
+    u64* dd = da + an;
+    hipStream_t st = D.stream;
+    HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
+    if ((rc = sqr_device(D, dd, da, an, st))) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst, dd, 2 * an * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NNH_OK;
+}
+
+int nnh_alpha(int scale, uint64_t out[3]) {
+    if (!out || scale < MIN_MID_BITS || scale > MAX_SCALE) return fail(NNH_ERR_ARG, "bad scale");
+    NEED_INIT();
+    for (int i = 0; i < 3; i++) out[i] = alpha_plain(g_dev, i, scale);
+    return NNH_OK;
+}
+
+int nnh_transform_forward(uint64_t* out, const uint64_t* a, uint64_t an, int scale) {
+    if (!out || !a || !an || scale < MIN_MID_BITS || scale > MAX_SCALE || an > (1ull << scale))
+        return fail(NNH_ERR_ARG, "bad transform arguments");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    Device& D = g_dev;
+    Plan pl = make_plan(D, scale);
+    int rc;
+    if ((rc = ensure_twiddles(D, scale)) || (rc = ensure(D.ws, 6 * pl.L)) || (rc = stage_host(D, an + 16)))
+        return rc;
+    hipStream_t st = D.stream;
+    HIPCHK(hipMemcpyAsync(D.stage.p, a, an * 8, hipMemcpyHostToDevice, st));
+    transform_full(D, pl, D.stage.p, an, D.ws.p, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, D.ws.p, 3 * pl.L * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NNH_OK;
+}
+
+int nnh_mul_transformed(uint64_t* dst, const uint64_t* fa, uint64_t an, const uint64_t* b, uint64_t bn,
+                        int scale) {
+    if (!dst || !fa || !b || !an || !bn || scale < MIN_MID_BITS || scale > MAX_SCALE ||
+        an + bn - 1 > (1ull << scale))
+        return fail(NNH_ERR_ARG, "bad arguments");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    Device& D = g_dev;
+    Plan pl = make_plan(D, scale);
+    int rc;
+    if ((rc = ensure_twiddles(D, scale)) || (rc = ensure(D.ws, 6 * pl.L)) ||
+        (rc = stage_host(D, bn + an + bn + 16)))
+        return rc;
+    hipStream_t st = D.stream;
+    u64* fsaved = D.ws.p + 3 * pl.L;  // the reused transform plays operand B
+    u64* db = D.stage.p;
+    u64* dd = db + bn;
+    HIPCHK(hipMemcpyAsync(fsaved, fa, 3 * pl.L * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
+    if ((rc = product_core(D, pl, db, bn, nullptr, an, false, fsaved, dd, an + bn, nullptr, 0, st))) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst, dd, (an + bn) * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NNH_OK;
+}
+
+int nnh_mul_cyclic(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn, int scale) {
+    if (!dst || !a || !b || !an || !bn || scale < MIN_MID_BITS || scale > MAX_SCALE)
+        return fail(NNH_ERR_ARG, "bad arguments");
+    u64 L = 1ull << scale;
+    if (an > L || bn > L) return fail(NNH_ERR_ARG, "operand longer than the transform");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    Device& D = g_dev;
+    Plan pl = make_plan(D, scale);
+    bool cyclic = an + bn - 1 > L;
+    u64 n_out = cyclic ? L + 2 : an + bn;
+    int rc;
+    if ((rc = ensure_twiddles(D, scale)) || (rc = ensure(D.ws, 6 * pl.L)) ||
+        (rc = stage_host(D, an + bn + n_out + 16)))
+        return rc;
+    hipStream_t st = D.stream;
+    u64* da = D.stage.p;
+    u64* db = da + an;
+    u64* dd = db + bn;
+    HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
+    if ((rc = product_core(D, pl, da, an, db, bn, false, nullptr, dd, n_out, nullptr, 0, st))) return rc;
+    if (cyclic) {
+        ProfScope ps(K_FOLD, 16 * L, st);
+        launch_cyclic_fold(dd, L, st);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst, dd, (cyclic ? L : an + bn) * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NNH_OK;
+}
+
+}  // extern "C"

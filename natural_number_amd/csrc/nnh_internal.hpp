@@ -1,0 +1,76 @@
+// nnh_internal.hpp — descriptors shared by the kernels and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace nnh {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+typedef uint8_t u8;
+
+// Root bases in Montgomery form (x * 2^64 mod p), by value in kernel args.
+struct RootBases {
+    u64 rb_m[3][56];
+    u64 irb_m[3][56];
+    u64 one_m[3];
+};
+
+// Garner constants (Montgomery form where they multiply residues).
+struct CrtConsts {
+    u64 c02m;   // m0^-1 mod m2, times 2^64 mod m2
+    u64 m0m;    // m0 mod m1, times 2^64 mod m1
+    u64 c1m;    // (m0 m2)^-1 mod m1, times 2^64 mod m1
+    u64 m02lo, m02hi;  // m0 * m2
+};
+
+struct TopPassDesc {
+    u64* data[3];
+    const u64* tw[3];
+    const u64* limbs;   // non-null: first forward pass loads from limbs
+    u64 an;
+    u64 alpha_m[3];
+    u64 L;
+    u32 g_lo, nlev;
+    bool inverse;
+};
+
+struct MidDesc {
+    u64* fa[3];
+    const u64* fb[3];
+    const u64* twf[3];
+    const u64* twi[3];
+    const u64* la;      // non-null: load limbs (L == 2^cb)
+    const u64* lb;
+    u64 an, bn;
+    u64 alpha_m[3];
+    u64 L;
+    int cb;
+    bool sqr;
+    bool fwd_only;   // store A's finished transform (no product)
+    int a_done;      // A already fully transformed
+    int b_done;
+};
+
+struct CrtDesc {
+    const u64* r[3];
+    int columns;
+    u64 n_res;
+    u64* out;
+    u64 n_out;
+    const u64* addend;
+    u64 n_add;
+    u8* tile_fn;
+    u8* tile_cin;
+    CrtConsts C;
+};
+
+void launch_twiddles(u64* tw, u64 n, const RootBases& rb, int pr, int inv, hipStream_t st);
+void launch_top_pass(const TopPassDesc& d, hipStream_t st);
+void launch_mid(const MidDesc& d, hipStream_t st);
+void launch_crt(const CrtDesc& d, hipStream_t st);
+u64 crt_tiles(u64 n_out);
+void launch_school_cols(const u64* a, u64 an, const u64* b, u64 bn, u64* c0, u64* c1, u64* c2, hipStream_t st);
+void launch_cyclic_fold(u64* w, u64 L, hipStream_t st);
+
+}  // namespace nnh

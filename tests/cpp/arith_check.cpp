@@ -1,0 +1,89 @@
+// Host-compiled check of the device arithmetic in nnh_arith.hpp (the same
+// __host__ __device__ functions the kernels use) against exact 128-bit math.
+// Built and run by tests/test_arith.py.  Exit status 0 = all checks pass.
+#include "../../natural_number_amd/csrc/nnh_arith.hpp"
+
+#include <cstdio>
+#include <random>
+
+using namespace nnh;
+typedef unsigned __int128 u128;
+
+static u64 powmod(u64 b, u64 e, u64 p) {
+    u64 r = 1;
+    for (b %= p; e; e >>= 1, b = (u64)((u128)b * b % p))
+        if (e & 1) r = (u64)((u128)r * b % p);
+    return r;
+}
+static u64 to_mont(u64 x, u64 p) { return (u64)(((u128)(x % p) << 64) % p); }
+
+template <u32 K>
+static int check_prime(std::mt19937_64& rng, int iters) {
+    const u64 p = prime_p(K);
+    const u64 rinv = powmod((u64)(((u128)1 << 64) % p), p - 2, p);  // 2^-64 mod p
+    int bad = 0;
+    auto expect = [&](u128 t) { return (u64)(t % p * rinv % p); };
+    for (int it = 0; it < iters; it++) {
+        u64 a, b;
+        switch (it % 4) {
+            case 0: a = rng() % (2 * p); b = rng() % p; break;             // butterfly operands
+            case 1: a = rng(); b = rng() % p; break;                       // limb * constant (load)
+            case 2: a = 2 * p - 1 - (rng() & 0xff); b = p - 1 - (rng() & 0xff); break;  // bound
+            default: a = rng() & 0xffff; b = rng() % p; break;
+        }
+        u64 r = mont<K>(a, b);
+        if (r >= 2 * p || r % p != expect((u128)a * b)) bad++;
+        u64 x0 = rng() % p, y0 = rng() % p, x1 = rng() % p, y1 = rng() % p;
+        u64 r2 = mont2<K>(x0, y0, x1, y1);
+        if (r2 >= 2 * p || r2 % p != expect((u128)x0 * y0 % p + (u128)x1 * y1 % p)) bad++;
+        u64 x = rng() % (2 * p);
+        if (csub<K>(x) != x % p) bad++;
+        // butterflies
+        u64 w = rng() % p, wm = to_mont(w, p);
+        u64 bx = rng() % (2 * p), by = rng() % (2 * p), cx = bx, cy = by;
+        ct_bfly<K>(cx, cy, wm);
+        u64 t = (u64)((u128)(by % p) * w % p);
+        if (cx >= 2 * p || cy >= 2 * p || cx % p != (bx % p + t) % p || cy % p != (bx % p + p - t) % p) bad++;
+        u64 gx = rng() % p, gy = rng() % p, ex = gx, ey = gy;
+        gs_bfly<K>(ex, ey, wm);
+        if (ex != (gx + gy) % p || ey != (u64)((u128)((gx + p - gy) % p) * w % p)) bad++;
+    }
+    if (redc<K>(0, 0) != 0) bad++;
+    std::printf("prime k=%u: %d failures in %d iterations\n", K, bad, iters);
+    return bad;
+}
+
+static int check_garner(std::mt19937_64& rng, int iters) {
+    const u64 m0 = prime_p(PRIME_K[0]), m1 = prime_p(PRIME_K[1]), m2 = prime_p(PRIME_K[2]);
+    u64 c02m = to_mont(powmod(m0, m2 - 2, m2), m2);
+    u64 m0m = to_mont(m0 % m1, m1);
+    u64 c1m = to_mont(powmod((u64)((u128)(m0 % m1) * (m2 % m1) % m1), m1 - 2, m1), m1);
+    u128 m02 = (u128)m0 * m2;
+    int bad = 0;
+    for (int it = 0; it < iters; it++) {
+        // random X < m0 m1 m2 built from random residues, or extreme residues
+        u64 r0 = rng() % m0, r1 = rng() % m1, r2 = rng() % m2;
+        if (it % 8 == 1) { r0 = m0 - 1; r1 = m1 - 1; r2 = m2 - 1; }
+        if (it % 8 == 2) { r0 = 0; r1 = 0; r2 = 0; }
+        if (it % 8 == 3) { r0 = m0 - 1; r1 = 0; r2 = m2 - 1; }
+        u64 x0, x1, x2;
+        garner(r0, r1, r2, c02m, m0m, c1m, (u64)m02, (u64)(m02 >> 64), x0, x1, x2);
+        // check X mod each prime via the 3-word value
+        auto modp = [&](u64 p) {
+            u128 t = ((u128)(x2 % p) << 64 | x1) % p;
+            return (u64)(((t << 64) | x0) % p);
+        };
+        if (modp(m0) != r0 || modp(m1) != r1 || modp(m2) != r2) bad++;
+        // X < m0 m1 m2 < 2^186: x2 < 2^58
+        if (x2 >> 58) bad++;
+    }
+    std::printf("garner: %d failures in %d iterations\n", bad, iters);
+    return bad;
+}
+
+int main() {
+    std::mt19937_64 rng(12345);
+    int bad = check_prime<PRIME_K[0]>(rng, 1000000) + check_prime<PRIME_K[1]>(rng, 1000000) +
+              check_prime<PRIME_K[2]>(rng, 1000000) + check_garner(rng, 1000000);
+    return bad ? 1 : 0;
+}
