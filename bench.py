@@ -45,6 +45,65 @@ def cpu_baseline(a, b):
     return out, time.perf_counter() - t
 
 
+def batch_mix(torch, nn, stream, reps=5):
+    """Config 3: 4096 uneven products; batched (bucketed persistent kernels,
+    one hipGraph) against one launch sequence per product on the same stream."""
+    from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
+
+    log("batch mix: generating operands")
+    sizes = batch_mix_sizes()
+    ops = list(batch_mix_operands(sizes))
+    host = np.concatenate([np.concatenate([a, b]) for a, b in ops])
+    inp = torch.from_numpy(host.view(np.int64)).cuda()
+    n_out = sum(a + b for a, b in sizes)
+    out_b = torch.empty(n_out, dtype=torch.int64, device="cuda")
+    out_s = torch.empty(n_out, dtype=torch.int64, device="cuda")
+    reqs_b, reqs_s = [], []
+    ia = oa = 0
+    for an, bn in sizes:
+        da, db = inp.data_ptr() + 8 * ia, inp.data_ptr() + 8 * (ia + an)
+        reqs_b.append((out_b.data_ptr() + 8 * oa, da, an, db, bn))
+        reqs_s.append((out_s.data_ptr() + 8 * oa, da, an, db, bn))
+        ia += an + bn
+        oa += an + bn
+    sp = stream.cuda_stream
+    log("batch mix: planning + graph capture")
+    b = nn.Batch(reqs_b, graph=True)
+    b.run(sp)
+    for r in reqs_s:
+        nn.mul_device_async(*r, sp)
+    torch.cuda.synchronize()
+    exact = bool(torch.equal(out_b, out_s))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        b.run(sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    tb = e0.elapsed_time(e1) / reps
+    e0.record(stream)
+    for _ in range(2):
+        for r in reqs_s:
+            nn.mul_device_async(*r, sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ts = e0.elapsed_time(e1) / 2
+    b.close()
+    n = len(sizes)
+    return {
+        "workload": "4096 products, operand bits log-uniform in [2^12, 2^24], half balanced, half up to 64:1 "
+                    "(generator seed 0x4E4E2E1972800003, splitmix64 operands)",
+        "products": n,
+        "batched_ms": tb,
+        "batched_products_per_s": n / (tb * 1e-3),
+        "per_product_launch_ms": ts,
+        "per_product_launch_products_per_s": n / (ts * 1e-3),
+        "speedup": ts / tb,
+        "bit_exact_batched_vs_single": exact,
+    }
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -61,7 +120,22 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-mix measurement")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration << 17
+        x = torch.empty(n, dtype=torch.int64, device="cuda").fill_(1)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               "(products shard across GPUs only in design, see the sharding notes)", file=sys.stderr)
@@ -73,14 +147,16 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     nn.lib()
+    stream = torch.cuda.Stream(device=dev)  # default priority; the library launches on it
+    torch.cuda.set_stream(stream)
     an = bn = 1 << 24
     a_h = nn.splitmix64(3, an)
     b_h = nn.splitmix64(4, bn)
     a = torch.from_numpy(a_h.view(np.int64)).to(dev)
     b = torch.from_numpy(b_h.view(np.int64)).to(dev)
     out = torch.empty(an + bn, dtype=torch.int64, device=dev)
-    stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
+    assert sp, "need a non-null HIP stream"
 
     def step():
         nn.mul_device_async(out.data_ptr(), a.data_ptr(), an, b.data_ptr(), bn, sp)
@@ -129,6 +205,8 @@ def main():
     s = 25
     L = 1 << s
     b_alg = 16 * (an + bn) + 144 * L
+
+    batch = None if args.no_batch else batch_mix(torch, nn, stream)
 
     res = out.cpu().numpy().view(np.uint64)
     gpu_digest = hashlib.sha256(res.tobytes()).hexdigest()
@@ -185,6 +263,7 @@ def main():
         },
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "batch_mix": batch,
         "bit_exact_vs_cpu": exact,
         "result_sha256": gpu_digest,
         "wall_s_timed": wall,

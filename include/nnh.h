@@ -78,6 +78,33 @@ int nnh_alpha(int scale, uint64_t out[3]);
  * an + bn - 1 <= 2^scale.  Host buffers. */
 int nnh_mul_transformed(uint64_t* dst, const uint64_t* fa, uint64_t an, const uint64_t* b, uint64_t bn, int scale);
 
+/* Batched products (no counterpart in the reference, which multiplies one
+ * pair at a time; needed for binary-splitting workloads such as calc_e,
+ * natural.h:841-857).  Requests carry DEVICE pointers; dst holds an+bn words.
+ * A request with a == b and an == bn is a squaring.  Products are bucketed
+ * by transform length; each transform stage is one persistent launch per
+ * bucket and one CRT/carry sequence serves the whole batch. */
+typedef struct nnh_req {
+    uint64_t* dst;
+    const uint64_t* a;
+    uint64_t an;
+    const uint64_t* b;
+    uint64_t bn;
+} nnh_req;
+
+typedef struct nnh_batch nnh_batch;
+
+/* Plan a batch: buckets, workspace, device job table.  flags bit 0: capture
+ * the launch sequence into a hipGraph replayed by nnh_batch_run.  The plan
+ * keeps the request pointers: rerunning recomputes from their current
+ * contents. */
+int nnh_batch_create(const nnh_req* reqs, uint64_t n, int flags, nnh_batch** out);
+/* Queue one execution of the batch on stream (NULL = library stream). */
+int nnh_batch_run(nnh_batch* b, void* stream);
+int nnh_batch_destroy(nnh_batch* b);
+/* One-shot: plan, run, synchronise, release. */
+int nnh_mul_batch(const nnh_req* reqs, uint64_t n, void* stream);
+
 /* Smallest operand size (words) at which nnh_mul uses the transform
  * (the reference's NTTmul_threshold, ntt_info.h:6). */
 uint64_t nnh_threshold_words(void);

@@ -13,7 +13,7 @@ import numpy as np
 __all__ = [
     "NNHError", "lib", "mul", "sqr", "mul_cyclic", "transform_forward", "alpha",
     "mul_transformed", "mul_device_async", "sqr_device_async", "threshold_words", "kernel_timing", "kernel_times",
-    "to_limbs", "from_limbs", "splitmix64",
+    "to_limbs", "from_limbs", "splitmix64", "Req", "Batch", "mul_batch",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -25,6 +25,12 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 
 class NNHError(RuntimeError):
     pass
+
+
+class Req(ctypes.Structure):
+    """struct nnh_req: device pointers and word counts of one product."""
+    _fields_ = [("dst", ctypes.c_void_p), ("a", ctypes.c_void_p), ("an", ctypes.c_uint64),
+                ("b", ctypes.c_void_p), ("bn", ctypes.c_uint64)]
 
 
 def lib():
@@ -52,6 +58,10 @@ def lib():
         "nnh_kernel_times": ([i32, ctypes.POINTER(ctypes.c_double), _u64p, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_char_p)], i32),
         "nnh_last_error": ([], ctypes.c_char_p),
+        "nnh_batch_create": ([ctypes.POINTER(Req), u64, i32, ctypes.POINTER(vp)], i32),
+        "nnh_batch_run": ([vp, vp], i32),
+        "nnh_batch_destroy": ([vp], i32),
+        "nnh_mul_batch": ([ctypes.POINTER(Req), u64, vp], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -144,6 +154,36 @@ def kernel_times():
         _check(lib().nnh_kernel_times(k, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(by), ctypes.byref(nm)))
         res[nm.value.decode()] = (ms.value, int(n.value), by.value)
     return res
+
+
+class Batch:
+    """A planned batch of device-pointer products (nnh_batch_create)."""
+
+    def __init__(self, reqs, graph=True):
+        arr = (Req * len(reqs))(*[Req(*r) for r in reqs])
+        h = ctypes.c_void_p()
+        _check(lib().nnh_batch_create(arr, len(reqs), 1 if graph else 0, ctypes.byref(h)))
+        self.h = h
+
+    def run(self, stream=None):
+        _check(lib().nnh_batch_run(self.h, stream))
+
+    def close(self):
+        if self.h:
+            _check(lib().nnh_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mul_batch(reqs, stream=None):
+    """One-shot batch of (dst, a, an, b, bn) device-pointer products."""
+    arr = (Req * len(reqs))(*[Req(*r) for r in reqs])
+    _check(lib().nnh_mul_batch(arr, len(reqs), stream))
 
 
 def threshold_words():

@@ -6,6 +6,7 @@ The library lands next to this file so it travels with the source tree.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -16,7 +17,7 @@ ARCH = os.environ.get("NNH_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function"]
 
-SOURCES = ["nnh_kernels.hip", "nnh_api.cpp"]
+SOURCES = ["nnh_top.hip", "nnh_mid.hip", "nnh_crt.hip", "nnh_api.cpp"]
 
 
 def _run(cmd):
@@ -27,19 +28,23 @@ def _run(cmd):
     return r
 
 
+def _compile(src, verbose):
+    obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+    cmd = ["hipcc", "-c", src, "-o", obj] + COMMON
+    if not src.endswith(".hip"):
+        cmd += ["-x", "hip"]
+    cmd += HIP_FLAGS
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    return obj
+
+
 def build(verbose=False):
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        cmd = ["hipcc", "-c", src, "-o", obj] + COMMON
-        if src.endswith(".hip"):
-            cmd += HIP_FLAGS
-        else:
-            cmd += ["-x", "hip"] + HIP_FLAGS
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        _run(cmd)
-        objs.append(obj)
+    """Compile every translation unit in parallel and link libnnh.so."""
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     tmp = LIB + ".tmp"
     _run(["hipcc", "-shared", "-o", tmp, f"--offload-arch={ARCH}"] + objs)
     os.replace(tmp, LIB)

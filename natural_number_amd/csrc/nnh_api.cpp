@@ -456,6 +456,195 @@ static int sqr_device(Device& D, u64* out, const u64* a, u64 an, hipStream_t st)
 
 static int stage_host(Device& D, u64 words) { return ensure(D.stage, words); }
 
+// ---------------------------------------------------------------------------
+// Batched products: jobs bucketed by transform scale, one device job table,
+// one workspace; enqueued directly or captured once into a hipGraph.
+// ---------------------------------------------------------------------------
+struct Bucket {
+    int s;
+    u32 first, count;
+};
+
+}  // namespace nnh
+
+struct nnh_batch {
+    std::vector<nnh::BJob> jobs;
+    std::vector<nnh::Bucket> buckets;
+    nnh::BJob* d_jobs = nullptr;
+    nnh::u32* d_sj = nullptr;
+    nnh::u32 n_school = 0;
+    nnh::u64 ncols = 0;
+    nnh::u64 ntiles = 0;
+    nnh::u8* d_tiles = nullptr;
+    nnh::u64* ws = nullptr;
+    nnh::u64 ws_words = 0;
+    int max_s = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+};
+
+namespace nnh {
+
+static void batch_free(nnh_batch* B) {
+    if (!B) return;
+    (void)hipDeviceSynchronize();
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->cap_stream) (void)hipStreamDestroy(B->cap_stream);
+    if (B->d_jobs) (void)hipFree(B->d_jobs);
+    if (B->d_sj) (void)hipFree(B->d_sj);
+    if (B->d_tiles) (void)hipFree(B->d_tiles);
+    if (B->ws) (void)hipFree(B->ws);
+    delete B;
+}
+
+// Enqueue every launch of the batch on st.
+static void batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
+    if (B->n_school) launch_school_batch(B->d_jobs, B->d_sj, B->n_school, B->ncols, st);
+    for (const Bucket& bk : B->buckets) {
+        Plan pl = make_plan(D, bk.s);
+        const BJob* jobs = B->d_jobs + bk.first;
+        TopBatchParams tp{};
+        for (int i = 0; i < 3; i++) tp.alpha_m[i] = pl.alpha_m[i];
+        tp.L = pl.L;
+        // forward top passes, both operands
+        std::vector<std::pair<int, int>> passes;
+        int g_hi = pl.s - 1, rem = pl.ntop;
+        while (rem > 0) {
+            int nl = std::min(rem, 8);
+            passes.push_back({g_hi - nl + 1, nl});
+            g_hi -= nl;
+            rem -= nl;
+        }
+        for (int op = 0; op < 2; op++)
+            for (size_t k = 0; k < passes.size(); k++) {
+                for (int i = 0; i < 3; i++) tp.tw[i] = D.twf[i];
+                tp.g_lo = (u32)passes[k].first;
+                tp.nlev = (u32)passes[k].second;
+                launch_top_pass_batch(jobs, bk.count, op, tp, false, k == 0, st);
+            }
+        MidBatchParams mp{};
+        for (int i = 0; i < 3; i++) {
+            mp.twf[i] = D.twf[i];
+            mp.twi[i] = D.twi[i];
+            mp.alpha_m[i] = pl.alpha_m[i];
+        }
+        mp.L = pl.L;
+        launch_mid_batch(jobs, bk.count, mp, pl.cb, pl.ntop == 0, st);
+        for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
+            for (int i = 0; i < 3; i++) tp.tw[i] = D.twi[i];
+            tp.g_lo = (u32)it->first;
+            tp.nlev = (u32)it->second;
+            launch_top_pass_batch(jobs, bk.count, 0, tp, true, false, st);
+        }
+    }
+    launch_crt_batch(B->d_jobs, (u32)B->jobs.size(), B->ntiles, B->d_tiles, B->d_tiles + B->ntiles, D.crt, st);
+}
+
+static int batch_create(Device& D, const nnh_req* reqs, u64 n, int flags, nnh_batch** out) {
+    nnh_batch* B = new nnh_batch();
+    struct Item {
+        u64 idx;
+        int s;  // 0 = schoolbook
+    };
+    std::vector<Item> items;
+    items.reserve(n);
+    for (u64 i = 0; i < n; i++) {
+        const nnh_req& r = reqs[i];
+        if (!r.dst || !r.a || !r.b || !r.an || !r.bn) {
+            delete B;
+            return fail(NNH_ERR_ARG, "batch request " + std::to_string(i) + ": null pointer or empty operand");
+        }
+        u64 mn = std::min(r.an, r.bn);
+        int s = mn < NTT_MUL_THRESHOLD ? 0 : ntt_scale(r.an, r.bn);
+        if (s > MAX_SCALE) {
+            delete B;
+            return fail(NNH_ERR_ARG, "batch request too large");
+        }
+        items.push_back({i, s});
+        B->max_s = std::max(B->max_s, s);
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.s < y.s; });
+    // workspace offsets
+    u64 ws = 0, tiles = 0, cols = 0;
+    for (const Item& it : items) {
+        const nnh_req& r = reqs[it.idx];
+        BJob J{};
+        bool sq = r.a == r.b && r.an == r.bn;
+        J.a = r.a;
+        J.b = r.b;
+        J.an = r.an;
+        J.bn = r.bn;
+        J.out = r.dst;
+        J.n_out = r.an + r.bn;
+        J.tile0 = tiles;
+        tiles += crt_tiles(J.n_out);
+        if (it.s == 0) {
+            J.school = 1;
+            J.n_res = r.an + r.bn - 1;
+            J.col0 = cols;
+            cols += J.n_res;
+            J.fa = (u64*)(uintptr_t)(ws * 8);  // offset, rebased below
+            ws += 3 * J.n_res;
+        } else {
+            u64 L = 1ull << it.s;
+            J.sqr = sq;
+            J.n_res = std::min<u64>(r.an + r.bn - 1, L);
+            J.fa = (u64*)(uintptr_t)(ws * 8);
+            J.fb = (u64*)(uintptr_t)((ws + 3 * L) * 8);
+            ws += 6 * L;
+            if (B->buckets.empty() || B->buckets.back().s != it.s)
+                B->buckets.push_back({it.s, (u32)B->jobs.size(), 0});
+            B->buckets.back().count++;
+        }
+        if (it.s == 0) B->n_school++;
+        B->jobs.push_back(J);
+    }
+    B->ncols = cols;
+    B->ntiles = tiles;
+    B->ws_words = ws;
+    int rc;
+    if (B->max_s && (rc = ensure_twiddles(D, B->max_s))) {
+        batch_free(B);
+        return rc;
+    }
+    auto alloc = [&](void** p, u64 bytes) -> int {
+        hipError_t e = hipMalloc(p, std::max<u64>(bytes, 64));
+        if (e != hipSuccess) return fail(NNH_ERR_NOMEM, std::string("batch allocation: ") + hipGetErrorString(e));
+        return NNH_OK;
+    };
+    if ((rc = alloc((void**)&B->ws, ws * 8)) || (rc = alloc((void**)&B->d_jobs, B->jobs.size() * sizeof(BJob))) ||
+        (rc = alloc((void**)&B->d_tiles, 2 * tiles)) || (rc = alloc((void**)&B->d_sj, B->n_school * 4 + 4))) {
+        batch_free(B);
+        return rc;
+    }
+    for (BJob& J : B->jobs) {
+        J.fa = B->ws + (uintptr_t)J.fa / 8;
+        if (!J.school) J.fb = B->ws + (uintptr_t)J.fb / 8;
+        if (J.school) {
+            for (int i = 0; i < 3; i++) J.r[i] = J.fa + (u64)i * J.n_res;
+        } else {
+            u64 L = (J.fb - J.fa) / 3;
+            for (int i = 0; i < 3; i++) J.r[i] = J.fa + (u64)i * L;
+            if (J.sqr) J.fb = J.fa;
+        }
+    }
+    std::vector<u32> sj(B->n_school);
+    for (u32 i = 0; i < B->n_school; i++) sj[i] = i;  // school jobs sort first
+    HIPCHK(hipMemcpy(B->d_jobs, B->jobs.data(), B->jobs.size() * sizeof(BJob), hipMemcpyHostToDevice));
+    if (B->n_school) HIPCHK(hipMemcpy(B->d_sj, sj.data(), sj.size() * 4, hipMemcpyHostToDevice));
+    if (flags & 1) {
+        HIPCHK(hipStreamCreateWithFlags(&B->cap_stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamBeginCapture(B->cap_stream, hipStreamCaptureModeThreadLocal));
+        batch_enqueue(D, B, B->cap_stream);
+        HIPCHK(hipStreamEndCapture(B->cap_stream, &B->graph));
+        HIPCHK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    }
+    *out = B;
+    return NNH_OK;
+}
+
 }  // namespace nnh
 
 // ===========================================================================
@@ -577,6 +766,43 @@ int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an) {
     HIPCHK(hipMemcpyAsync(dst, dd, 2 * an * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return NNH_OK;
+}
+
+int nnh_batch_create(const nnh_req* reqs, uint64_t n, int flags, nnh_batch** out) {
+    if (!reqs || !n || !out) return fail(NNH_ERR_ARG, "empty batch");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    *out = nullptr;
+    return batch_create(g_dev, reqs, n, flags, out);
+}
+
+int nnh_batch_run(nnh_batch* b, void* stream) {
+    if (!b) return fail(NNH_ERR_ARG, "null batch");
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
+    if (b->exec) HIPCHK(hipGraphLaunch(b->exec, st));
+    else batch_enqueue(g_dev, b, st);
+    HIPCHK(hipGetLastError());
+    return NNH_OK;
+}
+
+int nnh_batch_destroy(nnh_batch* b) {
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    batch_free(b);
+    return NNH_OK;
+}
+
+int nnh_mul_batch(const nnh_req* reqs, uint64_t n, void* stream) {
+    nnh_batch* b = nullptr;
+    int rc = nnh_batch_create(reqs, n, 0, &b);
+    if (rc) return rc;
+    rc = nnh_batch_run(b, stream);
+    if (!rc) {
+        hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
+        if (hipStreamSynchronize(st) != hipSuccess) rc = fail(NNH_ERR_HIP, "batch synchronisation failed");
+    }
+    nnh_batch_destroy(b);
+    return rc;
 }
 
 int nnh_alpha(int scale, uint64_t out[3]) {

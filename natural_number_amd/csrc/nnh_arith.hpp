@@ -32,103 +32,84 @@ __host__ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi
     lo = mk64(lo32(p00), lo32(p01));
 }
 
-// Montgomery reduction for p = K*2^56+1, R = 2^64: returns T * 2^-64 mod p
+// One NTT prime p = k*2^56 + 1 (held in scalar registers inside kernels).
+struct Prime {
+    u64 p;
+    u32 k;
+    u32 k24;  // k << 24
+};
+__host__ __device__ constexpr Prime make_prime(u32 k) { return Prime{prime_p(k), k, k << 24}; }
+__host__ __device__ constexpr Prime prime_of(int i) { return make_prime(PRIME_K[i]); }
+
+// Montgomery reduction for p = k*2^56+1, R = 2^64: returns T * 2^-64 mod p
 // in [0, 2p) for T = hi:lo < p * 2^64.
-//   m = T_lo * (K*2^56 - 1) mod 2^64 = ((T_lo*K mod 2^8) << 56) - T_lo
-//   (T + m*p) / 2^64 = T_hi + (e + m*K) / 2^8,  e = (T_lo + m) >> 56
+//   m = T_lo * (k*2^56 - 1) mod 2^64 = ((T_lo*k mod 2^8) << 56) - T_lo
+//   (T + m*p) / 2^64 = T_hi + (e + m*k) / 2^8,  e = (T_lo + m) >> 56
 // and T_lo + m = byte*2^56 + 2^64*[T_lo > byte*2^56].
-template <u32 K>
-__host__ __device__ __forceinline__ u64 redc(u64 lo, u64 hi) {
-    u32 byte24 = lo32(lo) * (K << 24);              // (T_lo*K mod 2^8) << 24
+__host__ __device__ __forceinline__ u64 redc(const Prime& P, u64 lo, u64 hi) {
+    u32 byte24 = lo32(lo) * P.k24;                   // (T_lo*k mod 2^8) << 24
     u64 top = (u64)byte24 << 32;                     // byte * 2^56
     u64 m = top - lo;
     u32 wrap = lo > top ? 1u : 0u;                   // carry of T_lo + m
-    u64 t = (u64)lo32(m) * K + (byte24 >> 24);       // < 2^40, low 8 bits cancel with e
-    u64 r = (u64)hi32(m) * (K << 24) + (t >> 8);
+    u64 t = (u64)lo32(m) * P.k + (byte24 >> 24);     // < 2^40, low 8 bits cancel with e
+    u64 r = (u64)hi32(m) * P.k24 + (t >> 8);
     return hi + r + wrap;
 }
 
-template <u32 K>
-__host__ __device__ __forceinline__ u64 mont(u64 a, u64 b) {
+__host__ __device__ __forceinline__ u64 mont(const Prime& P, u64 a, u64 b) {
     u64 lo, hi;
     mul_wide(a, b, lo, hi);
-    return redc<K>(lo, hi);
+    return redc(P, lo, hi);
 }
 
 // Sum of two products, reduced: (a*b + c*d) * 2^-64 mod p in [0, 2p);
 // requires a*b + c*d < p * 2^64.
-template <u32 K>
-__host__ __device__ __forceinline__ u64 mont2(u64 a, u64 b, u64 c, u64 d) {
+__host__ __device__ __forceinline__ u64 mont2(const Prime& P, u64 a, u64 b, u64 c, u64 d) {
     u64 l1, h1, l2, h2;
     mul_wide(a, b, l1, h1);
     mul_wide(c, d, l2, h2);
     u64 lo = l1 + l2;
     u64 hi = h1 + h2 + (lo < l1 ? 1u : 0u);
-    return redc<K>(lo, hi);
+    return redc(P, lo, hi);
 }
 
 // x in [0, 2p) -> [0, p)
-template <u32 K>
-__host__ __device__ __forceinline__ u64 csub(u64 x) {
-    constexpr u64 p = prime_p(K);
-    u64 y = x - p;
+__host__ __device__ __forceinline__ u64 csub(const Prime& P, u64 x) {
+    u64 y = x - P.p;
     return (i64)y < 0 ? x : y;
 }
 
 // Cooley-Tukey butterfly (x, y) -> (x + w*y, x - w*y), inputs and outputs in
 // [0, 2p); wm = w * 2^64 mod p.
-template <u32 K>
-__host__ __device__ __forceinline__ void ct_bfly(u64& x, u64& y, u64 wm) {
-    constexpr u64 p = prime_p(K);
-    u64 t = csub<K>(mont<K>(y, wm));
-    u64 a = csub<K>(x);
+__host__ __device__ __forceinline__ void ct_bfly(const Prime& P, u64& x, u64& y, u64 wm) {
+    u64 t = csub(P, mont(P, y, wm));
+    u64 a = csub(P, x);
     x = a + t;
-    y = a + p - t;
-}
-// w = 1 variant.
-template <u32 K>
-__host__ __device__ __forceinline__ void ct_bfly1(u64& x, u64& y) {
-    constexpr u64 p = prime_p(K);
-    u64 t = csub<K>(y);
-    u64 a = csub<K>(x);
-    x = a + t;
-    y = a + p - t;
+    y = a + P.p - t;
 }
 // Gentleman-Sande butterfly (x, y) -> (x + y, (x - y) * w), canonical in/out.
-template <u32 K>
-__host__ __device__ __forceinline__ void gs_bfly(u64& x, u64& y, u64 wm) {
-    constexpr u64 p = prime_p(K);
-    u64 a = csub<K>(x + y);
-    u64 d = x + p - y;
-    y = csub<K>(mont<K>(d, wm));
+__host__ __device__ __forceinline__ void gs_bfly(const Prime& P, u64& x, u64& y, u64 wm) {
+    u64 a = csub(P, x + y);
+    u64 d = x + P.p - y;
+    y = csub(P, mont(P, d, wm));
     x = a;
-}
-template <u32 K>
-__host__ __device__ __forceinline__ void gs_bfly1(u64& x, u64& y) {
-    constexpr u64 p = prime_p(K);
-    u64 a = csub<K>(x + y);
-    u64 d = csub<K>(x + p - y);
-    x = a;
-    y = d;
 }
 
 // Base multiply of one 4-word group (reference asmNttMul0/1/2,
 // asmLinux.asm:910-1367): (x0+x1X)(y0+y1X) mod (X^2 - w) and
 // (x2+x3X)(y2+y3X) mod (X^2 + w).  Inputs in [0, 2p); outputs canonical and
 // carrying a factor 2^-64 (absorbed by the load scaling).
-template <u32 K>
-__host__ __device__ __forceinline__ void base_mul4(u64& x0, u64& x1, u64& x2, u64& x3,
-                                          u64 y0, u64 y1, u64 y2, u64 y3, u64 wm) {
-    constexpr u64 p = prime_p(K);
-    x0 = csub<K>(x0); x1 = csub<K>(x1); x2 = csub<K>(x2); x3 = csub<K>(x3);
-    y0 = csub<K>(y0); y1 = csub<K>(y1); y2 = csub<K>(y2); y3 = csub<K>(y3);
-    u64 t1 = csub<K>(mont<K>(y1, wm));      // w*y1
-    u64 t3 = p - csub<K>(mont<K>(y3, wm));  // -w*y3, in (0, p]
-    u64 z0 = mont2<K>(x0, y0, x1, t1);
-    u64 z1 = mont2<K>(x0, y1, x1, y0);
-    u64 z2 = mont2<K>(x2, y2, x3, t3);
-    u64 z3 = mont2<K>(x2, y3, x3, y2);
-    x0 = csub<K>(z0); x1 = csub<K>(z1); x2 = csub<K>(z2); x3 = csub<K>(z3);
+__host__ __device__ __forceinline__ void base_mul4(const Prime& P, u64& x0, u64& x1, u64& x2, u64& x3, u64 y0,
+                                                   u64 y1, u64 y2, u64 y3, u64 wm) {
+    x0 = csub(P, x0); x1 = csub(P, x1); x2 = csub(P, x2); x3 = csub(P, x3);
+    y0 = csub(P, y0); y1 = csub(P, y1); y2 = csub(P, y2); y3 = csub(P, y3);
+    u64 t1 = csub(P, mont(P, y1, wm));        // w*y1
+    u64 t3 = P.p - csub(P, mont(P, y3, wm));  // -w*y3, in (0, p]
+    u64 z0 = mont2(P, x0, y0, x1, t1);
+    u64 z1 = mont2(P, x0, y1, x1, y0);
+    u64 z2 = mont2(P, x2, y2, x3, t3);
+    u64 z3 = mont2(P, x2, y3, x3, y2);
+    x0 = csub(P, z0); x1 = csub(P, z1); x2 = csub(P, z2); x3 = csub(P, z3);
 }
 
 // Garner reconstruction of X < p0*p1*p2 < 2^186 from canonical residues, in
@@ -138,16 +119,17 @@ __host__ __device__ __forceinline__ void base_mul4(u64& x0, u64& x1, u64& x2, u6
 // m02 = p0*p2 (two words).  Output X as three words x0 + x1 2^64 + x2 2^128.
 __host__ __device__ __forceinline__ void garner(u64 r0, u64 r1, u64 r2, u64 c02m, u64 m0m, u64 c1m, u64 m02lo,
                                                 u64 m02hi, u64& x0, u64& x1, u64& x2) {
-    constexpr u64 P0 = prime_p(PRIME_K[0]), P1 = prime_p(PRIME_K[1]), P2 = prime_p(PRIME_K[2]);
+    constexpr Prime Q0 = prime_of(0), Q1 = prime_of(1), Q2 = prime_of(2);
+    constexpr u64 P0 = Q0.p, P1 = Q1.p, P2 = Q2.p;
     u64 d = r2 + P2 - r0;                                  // (0, 2 p2)
-    u64 u = csub<PRIME_K[2]>(mont<PRIME_K[2]>(d, c02m));   // (r2 - r0)/p0 mod p2
+    u64 u = csub(Q2, mont(Q2, d, c02m));                   // (r2 - r0)/p0 mod p2
     u64 lo, hi;
     mul_wide(u, P0, lo, hi);
     lo += r0;
     hi += lo < r0 ? 1u : 0u;                               // x02 = r0 + p0 u
-    u64 tm = csub<PRIME_K[1]>(mont<PRIME_K[1]>(u, m0m));   // p0 u mod p1
+    u64 tm = csub(Q1, mont(Q1, u, m0m));                   // p0 u mod p1
     u64 e = r1 + (P1 - r0) + (P1 - tm);                    // r1 - x02 (mod p1), < 3 p1
-    u64 v = csub<PRIME_K[1]>(mont<PRIME_K[1]>(e, c1m));    // / (p0 p2) mod p1
+    u64 v = csub(Q1, mont(Q1, e, c1m));                    // / (p0 p2) mod p1
     u64 l1, h1, l2, h2;
     mul_wide(v, m02lo, l1, h1);
     mul_wide(v, m02hi, l2, h2);

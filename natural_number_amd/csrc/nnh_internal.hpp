@@ -65,6 +65,44 @@ struct CrtDesc {
     CrtConsts C;
 };
 
+// One product of a batch (device-resident table).
+struct BJob {
+    const u64* a;
+    const u64* b;
+    u64 an, bn;
+    u64* fa;          // transform workspace 3L (or school columns 3 n_res)
+    u64* fb;          // second transform 3L
+    const u64* r[3];  // CRT inputs: fa + i L, or the three column arrays
+    u64 n_res;        // positions with residues / columns
+    u64* out;
+    u64 n_out;
+    u64 tile0;        // first global CRT tile of this job
+    u64 col0;         // first global school column (school jobs)
+    u32 school;       // 1: schoolbook columns
+    u32 sqr;          // 1: b == a (single transform)
+};
+
+struct TopBatchParams {
+    const u64* tw[3];
+    u64 alpha_m[3];
+    u64 L;
+    u32 g_lo, nlev;
+};
+
+struct MidBatchParams {
+    const u64* twf[3];
+    const u64* twi[3];
+    u64 alpha_m[3];
+    u64 L;
+};
+
+void launch_top_pass_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
+                           bool from_limbs, hipStream_t st);
+void launch_mid_batch(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs, hipStream_t st);
+void launch_school_batch(const BJob* jobs, const u32* sj, u32 nsj, u64 ncols, hipStream_t st);
+void launch_crt_batch(const BJob* jobs, u32 njobs, u64 ntiles, u8* tile_fn, u8* tile_cin, const CrtConsts& C,
+                      hipStream_t st);
+
 void launch_twiddles(u64* tw, u64 n, const RootBases& rb, int pr, int inv, hipStream_t st);
 void launch_top_pass(const TopPassDesc& d, hipStream_t st);
 void launch_mid(const MidDesc& d, hipStream_t st);

@@ -17,9 +17,10 @@ static u64 powmod(u64 b, u64 e, u64 p) {
 }
 static u64 to_mont(u64 x, u64 p) { return (u64)(((u128)(x % p) << 64) % p); }
 
-template <u32 K>
-static int check_prime(std::mt19937_64& rng, int iters) {
-    const u64 p = prime_p(K);
+static int check_prime(std::mt19937_64& rng, int iters, int pi) {
+    const Prime P = prime_of(pi);
+    const u64 p = P.p;
+    const u32 K = P.k;
     const u64 rinv = powmod((u64)(((u128)1 << 64) % p), p - 2, p);  // 2^-64 mod p
     int bad = 0;
     auto expect = [&](u128 t) { return (u64)(t % p * rinv % p); };
@@ -31,30 +32,30 @@ static int check_prime(std::mt19937_64& rng, int iters) {
             case 2: a = 2 * p - 1 - (rng() & 0xff); b = p - 1 - (rng() & 0xff); break;  // bound
             default: a = rng() & 0xffff; b = rng() % p; break;
         }
-        u64 r = mont<K>(a, b);
+        u64 r = mont(P, a, b);
         if (r >= 2 * p || r % p != expect((u128)a * b)) bad++;
         u64 x0 = rng() % p, y0 = rng() % p, x1 = rng() % p, y1 = rng() % p;
-        u64 r2 = mont2<K>(x0, y0, x1, y1);
+        u64 r2 = mont2(P, x0, y0, x1, y1);
         if (r2 >= 2 * p || r2 % p != expect((u128)x0 * y0 % p + (u128)x1 * y1 % p)) bad++;
         u64 x = rng() % (2 * p);
-        if (csub<K>(x) != x % p) bad++;
+        if (csub(P, x) != x % p) bad++;
         // butterflies
         u64 w = rng() % p, wm = to_mont(w, p);
         u64 bx = rng() % (2 * p), by = rng() % (2 * p), cx = bx, cy = by;
-        ct_bfly<K>(cx, cy, wm);
+        ct_bfly(P, cx, cy, wm);
         u64 t = (u64)((u128)(by % p) * w % p);
         if (cx >= 2 * p || cy >= 2 * p || cx % p != (bx % p + t) % p || cy % p != (bx % p + p - t) % p) bad++;
         u64 gx = rng() % p, gy = rng() % p, ex = gx, ey = gy;
-        gs_bfly<K>(ex, ey, wm);
+        gs_bfly(P, ex, ey, wm);
         if (ex != (gx + gy) % p || ey != (u64)((u128)((gx + p - gy) % p) * w % p)) bad++;
     }
-    if (redc<K>(0, 0) != 0) bad++;
+    if (redc(P, 0, 0) != 0) bad++;
     std::printf("prime k=%u: %d failures in %d iterations\n", K, bad, iters);
     return bad;
 }
 
 static int check_garner(std::mt19937_64& rng, int iters) {
-    const u64 m0 = prime_p(PRIME_K[0]), m1 = prime_p(PRIME_K[1]), m2 = prime_p(PRIME_K[2]);
+    const u64 m0 = prime_of(0).p, m1 = prime_of(1).p, m2 = prime_of(2).p;
     u64 c02m = to_mont(powmod(m0, m2 - 2, m2), m2);
     u64 m0m = to_mont(m0 % m1, m1);
     u64 c1m = to_mont(powmod((u64)((u128)(m0 % m1) * (m2 % m1) % m1), m1 - 2, m1), m1);
@@ -83,7 +84,7 @@ static int check_garner(std::mt19937_64& rng, int iters) {
 
 int main() {
     std::mt19937_64 rng(12345);
-    int bad = check_prime<PRIME_K[0]>(rng, 1000000) + check_prime<PRIME_K[1]>(rng, 1000000) +
-              check_prime<PRIME_K[2]>(rng, 1000000) + check_garner(rng, 1000000);
+    int bad = check_prime(rng, 1000000, 0) + check_prime(rng, 1000000, 1) + check_prime(rng, 1000000, 2) +
+              check_garner(rng, 1000000);
     return bad ? 1 : 0;
 }
