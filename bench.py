@@ -45,7 +45,7 @@ def cpu_baseline(a, b):
     return out, time.perf_counter() - t
 
 
-def batch_mix(torch, nn, stream, reps=5):
+def batch_mix(nn, stream, reps=5):
     """Config 3: 4096 uneven products; batched (bucketed persistent kernels,
     one hipGraph) against one launch sequence per product on the same stream."""
     from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
@@ -54,42 +54,42 @@ def batch_mix(torch, nn, stream, reps=5):
     sizes = batch_mix_sizes()
     ops = list(batch_mix_operands(sizes))
     host = np.concatenate([np.concatenate([a, b]) for a, b in ops])
-    inp = torch.from_numpy(host.view(np.int64)).cuda()
+    inp = nn.DeviceWords(host.size, host)
     n_out = sum(a + b for a, b in sizes)
-    out_b = torch.empty(n_out, dtype=torch.int64, device="cuda")
-    out_s = torch.empty(n_out, dtype=torch.int64, device="cuda")
+    out_b = nn.DeviceWords(n_out)
+    out_s = nn.DeviceWords(n_out)
     reqs_b, reqs_s = [], []
     ia = oa = 0
     for an, bn in sizes:
-        da, db = inp.data_ptr() + 8 * ia, inp.data_ptr() + 8 * (ia + an)
-        reqs_b.append((out_b.data_ptr() + 8 * oa, da, an, db, bn))
-        reqs_s.append((out_s.data_ptr() + 8 * oa, da, an, db, bn))
+        da, db = inp.at(ia), inp.at(ia + an)
+        reqs_b.append((out_b.at(oa), da, an, db, bn))
+        reqs_s.append((out_s.at(oa), da, an, db, bn))
         ia += an + bn
         oa += an + bn
-    sp = stream.cuda_stream
+    sp = stream.ptr
     log("batch mix: planning + graph capture")
     b = nn.Batch(reqs_b, graph=True)
     b.run(sp)
     for r in reqs_s:
         nn.mul_device_async(*r, sp)
-    torch.cuda.synchronize()
-    exact = bool(torch.equal(out_b, out_s))
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+    nn.synchronize()
+    exact = bool(np.array_equal(out_b.download(), out_s.download()))
+    e0, e1 = nn.Event(), nn.Event()
+    nn.synchronize()
     e0.record(stream)
     for _ in range(reps):
         b.run(sp)
     e1.record(stream)
-    torch.cuda.synchronize()
-    tb = e0.elapsed_time(e1) / reps
+    tb = e0.elapsed_ms(e1) / reps
     e0.record(stream)
     for _ in range(2):
         for r in reqs_s:
             nn.mul_device_async(*r, sp)
     e1.record(stream)
-    torch.cuda.synchronize()
-    ts = e0.elapsed_time(e1) / 2
+    ts = e0.elapsed_ms(e1) / 2
     b.close()
+    for x in (inp, out_b, out_s):
+        x.free()
     n = len(sizes)
     return {
         "workload": "4096 products, operand bits log-uniform in [2^12, 2^24], half balanced, half up to 64:1 "
@@ -125,15 +125,15 @@ def main():
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration:
-        import torch
+        import natural_number_amd as nn
 
         n = args.copy_calibration << 17
-        x = torch.empty(n, dtype=torch.int64, device="cuda").fill_(1)
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
+        x = nn.DeviceWords(n, np.ones(n, dtype=np.uint64))
+        y = nn.DeviceWords(n)
+        nn.synchronize()
         for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
+            nn.lib().nnh_copy_device(y.ptr, x.ptr, 8 * n, None)
+        nn.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
         return
     if args.gpus != 1:
@@ -141,42 +141,38 @@ def main():
               "(products shard across GPUs only in design, see the sharding notes)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import natural_number_amd as nn
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    nn.lib()
-    stream = torch.cuda.Stream(device=dev)  # default priority; the library launches on it
-    torch.cuda.set_stream(stream)
+    if nn.lib().nnh_init(0) != 0:
+        raise SystemExit("nnh_init(0) failed: " + nn.lib().nnh_last_error().decode())
+    stream = nn.Stream()  # default priority; every product is launched on it
+    sp = stream.ptr
     an = bn = 1 << 24
     a_h = nn.splitmix64(3, an)
     b_h = nn.splitmix64(4, bn)
-    a = torch.from_numpy(a_h.view(np.int64)).to(dev)
-    b = torch.from_numpy(b_h.view(np.int64)).to(dev)
-    out = torch.empty(an + bn, dtype=torch.int64, device=dev)
-    sp = stream.cuda_stream
-    assert sp, "need a non-null HIP stream"
+    a = nn.DeviceWords(an, a_h)
+    b = nn.DeviceWords(bn, b_h)
+    out = nn.DeviceWords(an + bn)
 
     def step():
-        nn.mul_device_async(out.data_ptr(), a.data_ptr(), an, b.data_ptr(), bn, sp)
+        nn.mul_device_async(out.ptr, a.ptr, an, b.ptr, bn, sp)
 
     log("warmup")
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    nn.synchronize()
 
     log(f"timing {args.steps} steps")
-    torch.cuda.synchronize()
+    nn.synchronize()  # single GPU: the device-wide sync is the barrier
     t0 = time.perf_counter()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0, e1 = nn.Event(), nn.Event()
     e0.record(stream)
     for _ in range(args.steps):
         step()
     e1.record(stream)
-    torch.cuda.synchronize()
+    nn.synchronize()
     wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / args.steps
+    ms = e0.elapsed_ms(e1) / args.steps
     bits = float(an * 64)
     value = bits / (ms * 1e-3)
 
@@ -185,7 +181,7 @@ def main():
     nn.kernel_timing(True)
     for _ in range(args.steps):
         step()
-    torch.cuda.synchronize()
+    nn.synchronize()
     kt = nn.kernel_times()
     nn.kernel_timing(False)
     kernels = {}
@@ -206,9 +202,9 @@ def main():
     L = 1 << s
     b_alg = 16 * (an + bn) + 144 * L
 
-    batch = None if args.no_batch else batch_mix(torch, nn, stream)
+    batch = None if args.no_batch else batch_mix(nn, stream)
 
-    res = out.cpu().numpy().view(np.uint64)
+    res = out.download()
     gpu_digest = hashlib.sha256(res.tobytes()).hexdigest()
     cpu = None
     exact = None

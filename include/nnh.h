@@ -50,6 +50,21 @@ int nnh_mul(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, ui
 /* dst[0 .. 2an) = a^2.  Reference: natural::_sqr (natural.h:529-535). */
 int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an);
 
+/* Explicit algorithm arms (no counterpart in the reference, whose choice is
+ * fixed by thresholds, natural.h:444-450 and base.h:18,37).  Every arm is
+ * exact; `param` is the piece transform scale for NNH_ALGO_NTT_CHUNK
+ * (0 = smallest that holds the shorter operand twice).  Used by the
+ * autotuner and the cost model. */
+#define NNH_ALGO_AUTO 0        /* reference dispatch: schoolbook below 384 words, else NTT with one split */
+#define NNH_ALGO_SCHOOL 1      /* column-wise schoolbook (asmMul1) */
+#define NNH_ALGO_KARATSUBA 2   /* one Karatsuba level over schoolbook columns (toom22mul) */
+#define NNH_ALGO_NTT 3         /* one transform for the whole product */
+#define NNH_ALGO_NTT_SPLIT 4   /* the reference's one-level unbalanced split (_mul_NTT) */
+#define NNH_ALGO_NTT_CHUNK 5   /* larger operand in pieces, shorter operand's transform reused */
+int nnh_mul_algo(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn, int algo, int param);
+int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
+                              int algo, int param, void* stream);
+
 /* Device-pointer variants: operands and result already in HBM, work is
  * queued on `stream` (a hipStream_t, NULL = the library's stream) and the
  * call returns without synchronising. */
@@ -117,6 +132,23 @@ uint64_t nnh_threshold_words(void);
 #define NNH_KERNEL_KINDS 7
 int nnh_kernel_timing(int enable);
 int nnh_kernel_times(int kind, double* total_ms, uint64_t* launches, double* bytes, const char** name);
+
+/* Device plumbing (no counterpart in the reference): device memory, copies,
+ * streams and events through the library's own HIP runtime, so that a
+ * caller needs no second GPU runtime in its process.  Streams are created
+ * non-blocking at default priority; NULL stream = the library's stream. */
+int nnh_device_alloc(void** ptr, uint64_t bytes);
+int nnh_device_free(void* ptr);
+int nnh_copy_to_device(void* dst, const void* src, uint64_t bytes);
+int nnh_copy_to_host(void* dst, const void* src, uint64_t bytes);   /* synchronises the device first */
+int nnh_copy_device(void* dst, const void* src, uint64_t bytes, void* stream);
+int nnh_stream_create(void** stream);
+int nnh_stream_destroy(void* stream);
+int nnh_synchronize(void* stream);                                   /* NULL: whole device */
+int nnh_event_create(void** ev);
+int nnh_event_record(void* ev, void* stream);
+int nnh_event_elapsed_ms(void* start, void* stop, float* ms);
+int nnh_event_destroy(void* ev);
 
 /* Human-readable text of the last error on this thread. */
 const char* nnh_last_error(void);

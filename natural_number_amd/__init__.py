@@ -14,6 +14,7 @@ __all__ = [
     "NNHError", "lib", "mul", "sqr", "mul_cyclic", "transform_forward", "alpha",
     "mul_transformed", "mul_device_async", "sqr_device_async", "threshold_words", "kernel_timing", "kernel_times",
     "to_limbs", "from_limbs", "splitmix64", "Req", "Batch", "mul_batch",
+    "DeviceWords", "Stream", "Event", "synchronize",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -62,6 +63,20 @@ def lib():
         "nnh_batch_run": ([vp, vp], i32),
         "nnh_batch_destroy": ([vp], i32),
         "nnh_mul_batch": ([ctypes.POINTER(Req), u64, vp], i32),
+        "nnh_mul_algo": ([_u64p, _u64p, u64, _u64p, u64, i32, i32], i32),
+        "nnh_mul_algo_device_async": ([vp, vp, u64, vp, u64, i32, i32, vp], i32),
+        "nnh_device_alloc": ([ctypes.POINTER(vp), u64], i32),
+        "nnh_device_free": ([vp], i32),
+        "nnh_copy_to_device": ([vp, vp, u64], i32),
+        "nnh_copy_to_host": ([vp, vp, u64], i32),
+        "nnh_copy_device": ([vp, vp, u64, vp], i32),
+        "nnh_stream_create": ([ctypes.POINTER(vp)], i32),
+        "nnh_stream_destroy": ([vp], i32),
+        "nnh_synchronize": ([vp], i32),
+        "nnh_event_create": ([ctypes.POINTER(vp)], i32),
+        "nnh_event_record": ([vp, vp], i32),
+        "nnh_event_elapsed_ms": ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
+        "nnh_event_destroy": ([vp], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -93,6 +108,22 @@ def mul(a, b):
     out = np.empty(a.size + b.size, dtype=np.uint64)
     _check(lib().nnh_mul(_arr(out), _arr(a), a.size, _arr(b), b.size))
     return out
+
+
+ALGO_AUTO, ALGO_SCHOOL, ALGO_KARATSUBA, ALGO_NTT, ALGO_NTT_SPLIT, ALGO_NTT_CHUNK = range(6)
+ALGO_NAMES = ["auto", "school", "karatsuba", "ntt", "ntt_split", "ntt_chunk"]
+
+
+def mul_algo(a, b, algo, param=0):
+    """a * b through one explicit algorithm arm (exact whichever)."""
+    a, b = _limbs(a), _limbs(b)
+    out = np.empty(a.size + b.size, dtype=np.uint64)
+    _check(lib().nnh_mul_algo(_arr(out), _arr(a), a.size, _arr(b), b.size, algo, param))
+    return out
+
+
+def mul_algo_device_async(d_out, d_a, an, d_b, bn, algo, param=0, stream=None):
+    _check(lib().nnh_mul_algo_device_async(d_out, d_a, an, d_b, bn, algo, param, stream))
 
 
 def sqr(a):
@@ -184,6 +215,81 @@ def mul_batch(reqs, stream=None):
     """One-shot batch of (dst, a, an, b, bn) device-pointer products."""
     arr = (Req * len(reqs))(*[Req(*r) for r in reqs])
     _check(lib().nnh_mul_batch(arr, len(reqs), stream))
+
+
+class DeviceWords:
+    """A device buffer of 64-bit words owned through the library's runtime."""
+
+    def __init__(self, n, host=None):
+        self.n = int(n)
+        p = ctypes.c_void_p()
+        _check(lib().nnh_device_alloc(ctypes.byref(p), 8 * max(self.n, 1)))
+        self.ptr = p.value
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host, offset=0):
+        h = np.ascontiguousarray(host, dtype=np.uint64)
+        _check(lib().nnh_copy_to_device(self.ptr + 8 * offset, h.ctypes.data, 8 * h.size))
+
+    def download(self, n=None, offset=0):
+        n = self.n - offset if n is None else n
+        out = np.empty(n, dtype=np.uint64)
+        _check(lib().nnh_copy_to_host(out.ctypes.data, self.ptr + 8 * offset, 8 * n))
+        return out
+
+    def at(self, offset):
+        return self.ptr + 8 * offset
+
+    def free(self):
+        if self.ptr:
+            _check(lib().nnh_device_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _check(lib().nnh_stream_create(ctypes.byref(p)))
+        self.ptr = p.value
+
+    def synchronize(self):
+        _check(lib().nnh_synchronize(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            _check(lib().nnh_stream_destroy(self.ptr))
+            self.ptr = None
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _check(lib().nnh_event_create(ctypes.byref(p)))
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        _check(lib().nnh_event_record(self.ptr, stream.ptr if isinstance(stream, Stream) else stream))
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        _check(lib().nnh_event_elapsed_ms(self.ptr, end.ptr, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.ptr:
+            _check(lib().nnh_event_destroy(self.ptr))
+            self.ptr = None
+
+
+def synchronize(stream=None):
+    _check(lib().nnh_synchronize(stream.ptr if isinstance(stream, Stream) else stream))
 
 
 def threshold_words():

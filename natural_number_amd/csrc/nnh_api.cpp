@@ -239,17 +239,30 @@ static Plan make_plan(const Device& D, int s) {
     return pl;
 }
 
+// Top levels split into ceil(ntop/8) passes of nearly equal depth, deeper
+// first: deeper passes have narrower tiles (2^(12-nlev) columns), so
+// balancing keeps every row write at least 2^(12-7) words when ntop > 8.
+static std::vector<std::pair<int, int>> top_passes(const Plan& pl) {
+    std::vector<std::pair<int, int>> passes;  // (g_lo, nlev), forward order
+    int npass = (pl.ntop + 7) / 8;
+    int g_hi = pl.s - 1, rem = pl.ntop;
+    for (int k = 0; k < npass; k++) {
+        int nl = (rem + (npass - k) - 1) / (npass - k);
+        passes.push_back({g_hi - nl + 1, nl});
+        g_hi -= nl;
+        rem -= nl;
+    }
+    return passes;
+}
+
 static void set3(u64* dst[3], u64* base, u64 L) {
     for (int i = 0; i < 3; i++) dst[i] = base + (u64)i * L;
 }
 
 // Forward top levels (global bits s-1 .. 12) of operand x into f.
 static void fwd_top(const Device& D, const Plan& pl, const u64* x, u64 xn, u64* f, hipStream_t st) {
-    int g_hi = pl.s - 1;
     bool first = true;
-    int remaining = pl.ntop;
-    while (remaining > 0) {
-        int nl = std::min(remaining, 8);
+    for (auto [g_lo, nl] : top_passes(pl)) {
         TopPassDesc td{};
         set3(td.data, f, pl.L);
         for (int i = 0; i < 3; i++) {
@@ -259,7 +272,7 @@ static void fwd_top(const Device& D, const Plan& pl, const u64* x, u64 xn, u64* 
         td.limbs = first ? x : nullptr;
         td.an = xn;
         td.L = pl.L;
-        td.g_lo = (u32)(g_hi - nl + 1);
+        td.g_lo = (u32)g_lo;
         td.nlev = (u32)nl;
         td.inverse = false;
         {
@@ -268,22 +281,13 @@ static void fwd_top(const Device& D, const Plan& pl, const u64* x, u64 xn, u64* 
             ProfScope ps(first ? K_TOP_LOAD : K_TOP_FWD, rd + 24 * pl.L, st);
             launch_top_pass(td, st);
         }
-        g_hi -= nl;
-        remaining -= nl;
         first = false;
     }
 }
 
 // Inverse top levels (global bits 12 .. s-1), mirrored passes.
 static void inv_top(const Device& D, const Plan& pl, u64* f, hipStream_t st) {
-    std::vector<std::pair<int, int>> passes;  // (g_lo, nlev) in forward order
-    int g_hi = pl.s - 1, remaining = pl.ntop;
-    while (remaining > 0) {
-        int nl = std::min(remaining, 8);
-        passes.push_back({g_hi - nl + 1, nl});
-        g_hi -= nl;
-        remaining -= nl;
-    }
+    std::vector<std::pair<int, int>> passes = top_passes(pl);
     for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
         TopPassDesc td{};
         set3(td.data, f, pl.L);
@@ -444,6 +448,117 @@ static int mul_device(Device& D, u64* out, const u64* a, u64 an, const u64* b, u
     return product_core(D, pl, a + a0n, an - a0n, b, bn, false, fb, out + a0n, an - a0n + bn, D.tmp.p, bn, st);
 }
 
+// ---------------------------------------------------------------------------
+// Explicit algorithm arms (every arm is exact; the choice only costs time).
+// ---------------------------------------------------------------------------
+static int kara_device(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st) {
+    if (an < bn) {
+        std::swap(a, b);
+        std::swap(an, bn);
+    }
+    if (bn < 2) return school(D, a, an, b, bn, out, st);
+    u64 nc = an + bn - 1;
+    int rc = ensure(D.tmp, kara_ws_words(an, bn) + 3 * nc + 64);
+    if (rc) return rc;
+    u64* cols = D.tmp.p + kara_ws_words(an, bn);
+    {
+        ProfScope ps(K_SCHOOL, 8 * (an + bn) + 24 * nc, st);
+        launch_kara(a, an, b, bn, D.tmp.p, cols, st);
+    }
+    u64 n_out = an + bn;
+    u64 tiles = crt_tiles(n_out);
+    if ((rc = ensure(D.tiles, (2 * tiles + 7) / 8 + 16))) return rc;
+    CrtDesc cd{};
+    cd.r[0] = cols; cd.r[1] = cols + nc; cd.r[2] = cols + 2 * nc;
+    cd.columns = 1;
+    cd.n_res = nc;
+    cd.out = out;
+    cd.n_out = n_out;
+    cd.tile_fn = (u8*)D.tiles.p;
+    cd.tile_cin = (u8*)D.tiles.p + tiles;
+    cd.C = D.crt;
+    {
+        ProfScope ps(K_CRT, 24 * nc + 8 * n_out, st);
+        launch_crt(cd, st);
+    }
+    return NNH_OK;
+}
+
+// One transform of scale max(get_NTT_scale, 8) for the whole product.
+static int ntt_direct(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st) {
+    int s = std::max(ntt_scale(an, bn), MIN_MID_BITS);
+    if (s > MAX_SCALE) return fail(NNH_ERR_ARG, "operands too large");
+    Plan pl = make_plan(D, s);
+    int rc;
+    if ((rc = ensure_twiddles(D, s)) || (rc = ensure(D.ws, 6 * pl.L))) return rc;
+    return product_core(D, pl, a, an, b, bn, false, nullptr, out, an + bn, nullptr, 0, st);
+}
+
+// The larger operand cut into pieces of P = 2^sp - bn + 1 words, each
+// multiplied through a transform of scale sp that reuses b's transform;
+// piece products are accumulated through the CRT addend.
+static int ntt_chunked(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, int sp, hipStream_t st) {
+    if (an < bn) {
+        std::swap(a, b);
+        std::swap(an, bn);
+    }
+    if (sp <= 0) sp = ntt_scale(bn, bn);
+    sp = std::max(sp, MIN_MID_BITS);
+    if (sp > MAX_SCALE || (1ull << sp) < bn) return fail(NNH_ERR_ARG, "bad piece scale");
+    Plan pl = make_plan(D, sp);
+    u64 P = pl.L - bn + 1;
+    int rc;
+    if ((rc = ensure_twiddles(D, sp)) || (rc = ensure(D.ws, 6 * pl.L)) || (rc = ensure(D.tmp, bn + 64))) return rc;
+    u64* fb = D.ws.p + 3 * pl.L;
+    transform_full(D, pl, b, bn, fb, st);
+    for (u64 off = 0; off < an; off += P) {
+        u64 pn = std::min(P, an - off);
+        if (off == 0) {
+            if ((rc = product_core(D, pl, a, pn, b, bn, false, fb, out, pn + bn, nullptr, 0, st))) return rc;
+        } else {
+            HIPCHK(hipMemcpyAsync(D.tmp.p, out + off, bn * sizeof(u64), hipMemcpyDeviceToDevice, st));
+            if ((rc = product_core(D, pl, a + off, pn, b, bn, false, fb, out + off, pn + bn, D.tmp.p, bn, st)))
+                return rc;
+        }
+    }
+    return NNH_OK;
+}
+
+// The reference's dispatch without its schoolbook threshold: transform,
+// split once when halving the larger operand saves a power of two.
+static int ntt_refsplit(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st) {
+    if (an < bn) {
+        std::swap(a, b);
+        std::swap(an, bn);
+    }
+    u64 a0n = (an + 1) >> 1;
+    int s = ntt_scale(an, bn), s_opt = ntt_scale(a0n, bn);
+    if (s == s_opt || s_opt < MIN_MID_BITS || a0n < 1 || an - a0n < 1) return ntt_direct(D, out, a, an, b, bn, st);
+    Plan pl = make_plan(D, s_opt);
+    int rc;
+    if ((rc = ensure_twiddles(D, s_opt)) || (rc = ensure(D.ws, 6 * pl.L)) || (rc = ensure(D.tmp, bn + 64))) return rc;
+    u64* fb = D.ws.p + 3 * pl.L;
+    transform_full(D, pl, b, bn, fb, st);
+    if ((rc = product_core(D, pl, a, a0n, b, bn, false, fb, out, a0n + bn, nullptr, 0, st))) return rc;
+    HIPCHK(hipMemcpyAsync(D.tmp.p, out + a0n, bn * sizeof(u64), hipMemcpyDeviceToDevice, st));
+    return product_core(D, pl, a + a0n, an - a0n, b, bn, false, fb, out + a0n, an - a0n + bn, D.tmp.p, bn, st);
+}
+
+enum Algo { ALGO_AUTO = 0, ALGO_SCHOOL = 1, ALGO_KARATSUBA = 2, ALGO_NTT = 3, ALGO_NTT_SPLIT = 4, ALGO_NTT_CHUNK = 5 };
+
+static int mul_algo_device(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, int algo, int param,
+                           hipStream_t st) {
+    switch (algo) {
+        case ALGO_AUTO: return mul_device(D, out, a, an, b, bn, st);
+        case ALGO_SCHOOL: return an >= bn ? school(D, a, an, b, bn, out, st) : school(D, b, bn, a, an, out, st);
+        case ALGO_KARATSUBA: return kara_device(D, out, a, an, b, bn, st);
+        case ALGO_NTT: return ntt_direct(D, out, a, an, b, bn, st);
+        case ALGO_NTT_SPLIT: return ntt_refsplit(D, out, a, an, b, bn, st);
+        case ALGO_NTT_CHUNK: return ntt_chunked(D, out, a, an, b, bn, param, st);
+        default: return fail(NNH_ERR_ARG, "unknown algorithm");
+    }
+}
+
 static int sqr_device(Device& D, u64* out, const u64* a, u64 an, hipStream_t st) {
     if (an < NTT_MUL_THRESHOLD) return school(D, a, an, a, an, out, st);
     int s = ntt_scale(an, an);
@@ -509,14 +624,7 @@ static void batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
         for (int i = 0; i < 3; i++) tp.alpha_m[i] = pl.alpha_m[i];
         tp.L = pl.L;
         // forward top passes, both operands
-        std::vector<std::pair<int, int>> passes;
-        int g_hi = pl.s - 1, rem = pl.ntop;
-        while (rem > 0) {
-            int nl = std::min(rem, 8);
-            passes.push_back({g_hi - nl + 1, nl});
-            g_hi -= nl;
-            rem -= nl;
-        }
+        std::vector<std::pair<int, int>> passes = top_passes(pl);
         for (int op = 0; op < 2; op++)
             for (size_t k = 0; k < passes.size(); k++) {
                 for (int i = 0; i < 3; i++) tp.tw[i] = D.twf[i];
@@ -750,6 +858,38 @@ int nnh_mul(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, ui
     return NNH_OK;
 }
 
+int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
+                              int algo, int param, void* stream) {
+    if (!d_dst || !d_a || !d_b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
+    int rc = mul_algo_device(g_dev, d_dst, d_a, an, d_b, bn, algo, param, st);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    return NNH_OK;
+}
+
+int nnh_mul_algo(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn, int algo, int param) {
+    if (!dst || !a || !b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    Device& D = g_dev;
+    int rc = stage_host(D, 2 * (an + bn) + 16);
+    if (rc) return rc;
+    u64* da = D.stage.p;
+    u64* db = da + an;
+    u64* dd = db + bn;
+    hipStream_t st = D.stream;
+    HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
+    if ((rc = mul_algo_device(D, dd, da, an, db, bn, algo, param, st))) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst, dd, (an + bn) * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NNH_OK;
+}
+
 int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an) {
     if (!dst || !a || !an) return fail(NNH_ERR_ARG, "null pointer or empty operand");
     NEED_INIT();
@@ -765,6 +905,76 @@ int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(dst, dd, 2 * an * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return NNH_OK;
+}
+
+// ---- device plumbing (so callers need no second HIP runtime) ----------------
+int nnh_device_alloc(void** ptr, uint64_t bytes) {
+    if (!ptr) return fail(NNH_ERR_ARG, "null pointer");
+    NEED_INIT();
+    *ptr = nullptr;
+    HIPCHK(hipMalloc(ptr, bytes ? bytes : 8));
+    return NNH_OK;
+}
+int nnh_device_free(void* ptr) {
+    if (!ptr) return NNH_OK;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(ptr));
+    return NNH_OK;
+}
+int nnh_copy_to_device(void* dst, const void* src, uint64_t bytes) {
+    NEED_INIT();
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return NNH_OK;
+}
+int nnh_copy_to_host(void* dst, const void* src, uint64_t bytes) {
+    NEED_INIT();
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return NNH_OK;
+}
+int nnh_copy_device(void* dst, const void* src, uint64_t bytes, void* stream) {
+    NEED_INIT();
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream ? (hipStream_t)stream : g_dev.stream));
+    return NNH_OK;
+}
+int nnh_stream_create(void** stream) {
+    if (!stream) return fail(NNH_ERR_ARG, "null pointer");
+    NEED_INIT();
+    hipStream_t st = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+    *stream = st;
+    return NNH_OK;
+}
+int nnh_stream_destroy(void* stream) {
+    if (stream) HIPCHK(hipStreamDestroy((hipStream_t)stream));
+    return NNH_OK;
+}
+int nnh_synchronize(void* stream) {
+    NEED_INIT();
+    if (stream) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    else HIPCHK(hipDeviceSynchronize());
+    return NNH_OK;
+}
+int nnh_event_create(void** ev) {
+    if (!ev) return fail(NNH_ERR_ARG, "null pointer");
+    NEED_INIT();
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreate(&e));
+    *ev = e;
+    return NNH_OK;
+}
+int nnh_event_record(void* ev, void* stream) {
+    HIPCHK(hipEventRecord((hipEvent_t)ev, stream ? (hipStream_t)stream : g_dev.stream));
+    return NNH_OK;
+}
+int nnh_event_elapsed_ms(void* start, void* stop, float* ms) {
+    HIPCHK(hipEventSynchronize((hipEvent_t)stop));
+    HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return NNH_OK;
+}
+int nnh_event_destroy(void* ev) {
+    if (ev) HIPCHK(hipEventDestroy((hipEvent_t)ev));
     return NNH_OK;
 }
 

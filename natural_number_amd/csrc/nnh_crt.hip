@@ -2,6 +2,8 @@
 // columns, the cyclic fold and the twiddle tables.
 #include "nnh_tile.hpp"
 
+#include <algorithm>
+
 namespace nnh {
 
 // ---------------------------------------------------------------------------
@@ -306,6 +308,99 @@ __global__ __launch_bounds__(256) void k_school_cols(const u64* __restrict__ a, 
     c0[k] = s0; c1[k] = s1; c2[k] = s2;
 }
 
+// ---------------------------------------------------------------------------
+// One Karatsuba level in the polynomial domain (the reference's toom22mul,
+// base.h:21-35, without its carry passes): with h = ceil(bn/2),
+//   a = a0 + a1 x^h, b = b0 + b1 x^h (x = 2^64),
+//   ab = z0 + (zm - z0 - z2) x^h + z2 x^2h,  zm = (a0 + a1)(b0 + b1),
+// where every term is a vector of column sums.  a0 + a1 is formed
+// coefficient-wise (65-bit coefficients, no carry chain), so zm's columns
+// stay below 2^192 and the combined columns C_k = sum a_i b_{k-i} (three
+// words, non-negative) go through the same carry kernel as schoolbook.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_kara_prep(const u64* __restrict__ a, u64 an, const u64* __restrict__ b,
+                                                   u64 bn, u64 h, u64* s_lo, u64* s_hi, u64 ns, u64* t_lo,
+                                                   u64* t_hi, u64 nt) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ns) {
+        u64 x = i < h ? a[i] : 0, y = i < an - h ? a[h + i] : 0;
+        u64 z = x + y;
+        s_lo[i] = z;
+        s_hi[i] = z < x ? 1u : 0u;
+    }
+    if (i < nt) {
+        u64 x = i < h ? b[i] : 0, y = i < bn - h ? b[h + i] : 0;
+        u64 z = x + y;
+        t_lo[i] = z;
+        t_hi[i] = z < x ? 1u : 0u;
+    }
+}
+
+// Column sums of two polynomials with 65-bit coefficients (lo word + 0/1).
+__global__ __launch_bounds__(256) void k_cols_wide(const u64* __restrict__ x_lo, const u64* __restrict__ x_hi,
+                                                   u64 nx, const u64* __restrict__ y_lo,
+                                                   const u64* __restrict__ y_hi, u64 ny, u64* c0, u64* c1,
+                                                   u64* c2) {
+    u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nx + ny - 1) return;
+    u64 i0 = k >= ny ? k - ny + 1 : 0;
+    u64 i1 = k < nx - 1 ? k : nx - 1;
+    u64 s0 = 0, s1 = 0, s2 = 0;
+    for (u64 i = i0; i <= i1; i++) {
+        u64 xl = x_lo[i], xh = x_hi[i], yl = y_lo[k - i], yh = y_hi[k - i];
+        u64 lo, hi;
+        mul_wide(xl, yl, lo, hi);
+        // + (xh*yl + yh*xl) 2^64 + xh*yh 2^128
+        u64 top = xh & yh;
+        u64 m1 = xh ? yl : 0, m2 = yh ? xl : 0;
+        u64 h1 = hi + m1;
+        top += h1 < m1 ? 1u : 0u;
+        u64 h2 = h1 + m2;
+        top += h2 < m2 ? 1u : 0u;
+        s0 += lo;
+        u64 c = s0 < lo ? 1u : 0u;
+        u64 hh = h2 + c;
+        top += hh < c ? 1u : 0u;
+        s1 += hh;
+        s2 += top + (s1 < hh ? 1u : 0u);
+    }
+    c0[k] = s0; c1[k] = s1; c2[k] = s2;
+}
+
+__device__ __forceinline__ void add3(u64& a0, u64& a1, u64& a2, u64 b0, u64 b1, u64 b2) {
+    u64 r0 = a0 + b0;
+    u64 c = r0 < a0 ? 1u : 0u;
+    u64 t = a1 + b1;
+    u64 c1 = t < a1 ? 1u : 0u;
+    u64 r1 = t + c;
+    c1 += r1 < t ? 1u : 0u;
+    a0 = r0; a1 = r1; a2 = a2 + b2 + c1;
+}
+__device__ __forceinline__ void sub3(u64& a0, u64& a1, u64& a2, u64 b0, u64 b1, u64 b2) {
+    u64 r0 = a0 - b0;
+    u64 br = a0 < b0 ? 1u : 0u;
+    u64 t = a1 - b1;
+    u64 br1 = a1 < b1 ? 1u : 0u;
+    u64 r1 = t - br;
+    br1 += t < br ? 1u : 0u;
+    a0 = r0; a1 = r1; a2 = a2 - b2 - br1;
+}
+
+// C_k = z0_k + zm_{k-h} - z0_{k-h} - z2_{k-h} + z2_{k-2h}, arithmetic mod 2^192
+// (the true column is in [0, 2^192)).  z arrays are three word planes.
+__global__ __launch_bounds__(256) void k_kara_combine(const u64* z0, u64 n0, const u64* zm, u64 nm, const u64* z2,
+                                                      u64 n2, u64 h, u64* c, u64 nc) {
+    u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    u64 r0 = 0, r1 = 0, r2 = 0;
+    if (k < n0) add3(r0, r1, r2, z0[k], z0[n0 + k], z0[2 * n0 + k]);
+    if (k >= h && k - h < nm) add3(r0, r1, r2, zm[k - h], zm[nm + k - h], zm[2 * nm + k - h]);
+    if (k >= h && k - h < n0) sub3(r0, r1, r2, z0[k - h], z0[n0 + k - h], z0[2 * n0 + k - h]);
+    if (k >= h && k - h < n2) sub3(r0, r1, r2, z2[k - h], z2[n2 + k - h], z2[2 * n2 + k - h]);
+    if (k >= 2 * h && k - 2 * h < n2) add3(r0, r1, r2, z2[k - 2 * h], z2[n2 + k - 2 * h], z2[2 * n2 + k - 2 * h]);
+    c[k] = r0; c[nc + k] = r1; c[2 * nc + k] = r2;
+}
+
 // Cyclic fold (reference NTT_info::save, ntt_info.h:238-242): w[0..L) +=
 // (w[L], w[L+1]) with carry, then the carry out is added once more at word 0
 // (its own carry out is dropped, as in the reference).  One work-group.
@@ -379,6 +474,32 @@ void launch_school_cols(const u64* a, u64 an, const u64* b, u64 bn, u64* c0, u64
     u64 n = an + bn - 1;
     hipLaunchKernelGGL(k_school_cols, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, an, b, bn, c0, c1,
                        c2);
+}
+
+void launch_kara(const u64* a, u64 an, const u64* b, u64 bn, u64* ws, u64* cols, hipStream_t st) {
+    // an >= bn >= 2; h = ceil(bn/2); workspace layout documented in nnh_internal.hpp
+    u64 h = (bn + 1) / 2;
+    u64 ns = std::max(h, an - h), nt = h;
+    u64 *s_lo = ws, *s_hi = s_lo + ns, *t_lo = s_hi + ns, *t_hi = t_lo + nt;
+    u64 n0 = 2 * h - 1, n2 = (an - h) + (bn - h) - 1, nm = ns + nt - 1;
+    u64 *z0 = t_hi + nt, *z2 = z0 + 3 * n0, *zm = z2 + 3 * n2;
+    u64 m = std::max(ns, nt);
+    hipLaunchKernelGGL(k_kara_prep, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, a, an, b, bn, h, s_lo, s_hi,
+                       ns, t_lo, t_hi, nt);
+    launch_school_cols(a, h, b, h, z0, z0 + n0, z0 + 2 * n0, st);
+    launch_school_cols(a + h, an - h, b + h, bn - h, z2, z2 + n2, z2 + 2 * n2, st);
+    hipLaunchKernelGGL(k_cols_wide, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, st, s_lo, s_hi, ns, t_lo, t_hi,
+                       nt, zm, zm + nm, zm + 2 * nm);
+    u64 nc = an + bn - 1;
+    hipLaunchKernelGGL(k_kara_combine, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, z0, n0, zm, nm, z2, n2, h,
+                       cols, nc);
+}
+
+u64 kara_ws_words(u64 an, u64 bn) {
+    u64 h = (bn + 1) / 2;
+    u64 ns = std::max(h, an - h), nt = h;
+    u64 n0 = 2 * h - 1, n2 = (an - h) + (bn - h) - 1, nm = ns + nt - 1;
+    return 2 * ns + 2 * nt + 3 * (n0 + n2 + nm);
 }
 
 void launch_cyclic_fold(u64* w, u64 L, hipStream_t st) {

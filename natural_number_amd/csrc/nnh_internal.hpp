@@ -110,5 +110,10 @@ void launch_crt(const CrtDesc& d, hipStream_t st);
 u64 crt_tiles(u64 n_out);
 void launch_school_cols(const u64* a, u64 an, const u64* b, u64 bn, u64* c0, u64* c1, u64* c2, hipStream_t st);
 void launch_cyclic_fold(u64* w, u64 L, hipStream_t st);
+// One Karatsuba level over schoolbook columns (an >= bn >= 2): writes the
+// three column planes of a*b (an+bn-1 columns each) into cols; needs
+// kara_ws_words(an, bn) words of workspace.
+void launch_kara(const u64* a, u64 an, const u64* b, u64 bn, u64* ws, u64* cols, hipStream_t st);
+u64 kara_ws_words(u64 an, u64 bn);
 
 }  // namespace nnh

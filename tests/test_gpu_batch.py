@@ -13,32 +13,32 @@ def say(*a):
 
 
 class DevBatch:
-    """Operands packed into one device buffer, outputs into another."""
+    """Operands packed into one device buffer, outputs into another (device
+    memory through the library's own runtime)."""
 
-    def __init__(self, torch, ops):
-        self.torch = torch
+    def __init__(self, nnh, ops):
         sizes = [(a.size, b.size) for a, b in ops]
-        n_in = sum(an + bn for an, bn in sizes)
+        host = np.concatenate([np.concatenate([a, b]) for a, b in ops])
         n_out = sum(an + bn for an, bn in sizes)
-        host = np.concatenate([np.concatenate([a, b]) for a, b in ops]) if ops else np.zeros(0, np.uint64)
-        self.inp = torch.from_numpy(host.view(np.int64)).cuda()
-        self.out = torch.zeros(n_out, dtype=torch.int64, device="cuda")
+        self.inp = nnh.DeviceWords(host.size, host)
+        self.out = nnh.DeviceWords(n_out)
+        self.zeros = np.zeros(n_out, dtype=np.uint64)
         self.reqs, self.spans = [], []
-        ip, op = self.inp.data_ptr(), self.out.data_ptr()
         ia = oa = 0
         for (a, b), (an, bn) in zip(ops, sizes):
-            da, db = ip + 8 * ia, ip + 8 * (ia + an)
+            da, db = self.inp.at(ia), self.inp.at(ia + an)
             if b is a:
-                db, bn_ = da, an
-            else:
-                bn_ = bn
-            self.reqs.append((op + 8 * oa, da, an, db, bn_))
+                db = da
+            self.reqs.append((self.out.at(oa), da, an, db, bn))
             self.spans.append((oa, an + bn))
             ia += an + bn
             oa += an + bn
 
+    def clear(self):
+        self.out.upload(self.zeros)
+
     def results(self):
-        o = self.out.cpu().numpy().view(np.uint64)
+        o = self.out.download()
         return [o[s:s + n] for s, n in self.spans]
 
 
@@ -53,15 +53,7 @@ def make_ops(rng, sizes, square_every=0):
     return ops
 
 
-@pytest.fixture(scope="module")
-def torch():
-    import torch as t
-
-    assert t.cuda.is_available()
-    return t
-
-
-def test_batch_mixed_vs_oracle(nnh, oracle, torch):
+def test_batch_mixed_vs_oracle(nnh, oracle):
     rng = np.random.default_rng(77)
     sizes = []
     for _ in range(160):
@@ -70,25 +62,23 @@ def test_batch_mixed_vs_oracle(nnh, oracle, torch):
         sizes.append((an, bn))
     sizes += [(384, 384), (383, 9000), (4096, 4096), (4097, 100), (30000, 20000)]
     ops = make_ops(rng, sizes, square_every=9)
-    db = DevBatch(torch, ops)
+    db = DevBatch(nnh, ops)
     for graph in (False, True):
         say(f"batch of {len(ops)} graph={graph}")
-        db.out.zero_()
-        torch.cuda.synchronize()
+        db.clear()
         b = nnh.Batch(db.reqs, graph=graph)
         for rep in range(3 if graph else 1):
             if rep:
-                db.out.zero_()
-                torch.cuda.synchronize()
+                db.clear()
             b.run()
-            torch.cuda.synchronize()
+            nnh.synchronize()
             for (a, bb), got in zip(ops, db.results()):
                 exp = oracle.sqr(a) if bb is a else oracle.mul(a, bb)
                 assert np.array_equal(got, exp), (a.size, bb.size, graph, rep)
         b.close()
 
 
-def test_batch_config3_matches_single_path(nnh, torch):
+def test_batch_config3_matches_single_path(nnh):
     """All 4096 products of the uneven mix: batched (graph) == one call per product,
     and every product checked modulo two primes independently of the GPU."""
     from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
@@ -96,16 +86,15 @@ def test_batch_config3_matches_single_path(nnh, torch):
     sizes = batch_mix_sizes()
     ops = list(batch_mix_operands(sizes))
     say(f"config3 batch: {len(ops)} products, {sum(a + b for a, b in sizes)} words")
-    db = DevBatch(torch, ops)
+    db = DevBatch(nnh, ops)
     b = nnh.Batch(db.reqs, graph=True)
     b.run()
-    torch.cuda.synchronize()
+    nnh.synchronize()
     batched = [r.copy() for r in db.results()]
-    db.out.zero_()
-    torch.cuda.synchronize()
+    db.clear()
     for r in db.reqs:
         nnh.mul_device_async(*r, None)
-    torch.cuda.synchronize()
+    nnh.synchronize()
     single = db.results()
     for i, (x, y) in enumerate(zip(batched, single)):
         assert np.array_equal(x, y), i
