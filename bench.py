@@ -104,6 +104,48 @@ def batch_mix(nn, stream, reps=5):
     }
 
 
+def cost_model_run(n_side, n_held):
+    """Config 5: timings over a log-uniform (m, n) grid of 2^10 .. 2^30-bit
+    operands and batch counts, the FP32 MLP trained on the GPU by the
+    library's kernels, and its regret against exhaustive autotune on held-out
+    log-uniform requests (generator seed 0x4E4E2E1972800005)."""
+    import natural_number_amd as nn
+    from natural_number_amd import costmodel as cm
+    from natural_number_amd.workloads import log_uniform_requests
+
+    if nn.lib().nnh_init(0) != 0:
+        raise SystemExit("nnh_init(0) failed")
+    t0 = time.perf_counter()
+    bench = cm.Bench()
+    pts = cm.grid(n_side)
+    log(f"cost model: timing {len(pts)} grid points x {cm.N_ARMS} arms")
+    X, T, M = cm.collect(bench, pts, log=log)
+    t_collect = time.perf_counter() - t0
+    model = cm.CostModel(seed=7)
+    t1 = time.perf_counter()
+    loss0 = model.train(X, T, M, steps=1)
+    loss = model.train(X, T, M, steps=4000, lr=3e-3)
+    t_train = time.perf_counter() - t1
+    reqs = log_uniform_requests(n_held, 0x4E4E2E1972800005)
+    log(f"cost model: evaluating {len(reqs)} held-out requests")
+    t2 = time.perf_counter()
+    ev = cm.evaluate(bench, model, reqs, log=log)
+    t_eval = time.perf_counter() - t2
+    bench.close()
+    model.close()
+    ev.update({
+        "arms": cm.ARMS,
+        "grid_points": len(pts),
+        "train_loss_initial": loss0,
+        "train_loss_final": loss,
+        "train_rmse_log2_ms": loss ** 0.5,
+        "seconds_collect": t_collect,
+        "seconds_train": t_train,
+        "seconds_evaluate": t_eval,
+    })
+    return {"cost_model": ev}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -121,9 +163,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-mix measurement")
+    ap.add_argument("--cost-model", action="store_true",
+                    help="only run the cost-model pipeline (grid timings, on-device training, held-out regret)")
+    ap.add_argument("--held-out", type=int, default=10000, help="held-out requests for --cost-model")
+    ap.add_argument("--grid", type=int, default=20, help="grid points per side for --cost-model")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.cost_model:
+        print(json.dumps(cost_model_run(args.grid, args.held_out)), flush=True)
+        return
     if args.copy_calibration:
         import natural_number_amd as nn
 

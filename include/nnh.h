@@ -65,6 +65,35 @@ int nnh_mul_algo(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* 
 int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
                               int algo, int param, void* stream);
 
+/* Autotuner timing: average device milliseconds per product of `count`
+ * back-to-back products through one arm, over `reps` repetitions after one
+ * warm-up, on the library stream (device pointers). */
+int nnh_time_algo(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn, int algo,
+                  int param, int count, int reps, float* ms_per_product);
+
+/* Learned cost model (no counterpart in the reference): an FP32 MLP
+ * [n_in -> 64 -> 64 -> n_out] with ReLU, trained on the device by
+ * hand-written kernels (forward, backward, Adam) on masked mean-squared
+ * error; outputs are predicted log2 times of the candidate arms.  Parameter
+ * layout (flat float32): W1[64][n_in], b1[64], W2[64][64], b2[64],
+ * W3[n_out][64], b3[n_out]. */
+typedef struct nnh_mlp nnh_mlp;
+int nnh_mlp_create(int n_in, int n_out, uint64_t seed, nnh_mlp** out);
+int nnh_mlp_destroy(nnh_mlp* m);
+int nnh_mlp_num_params(nnh_mlp* m);
+int nnh_mlp_set_params(nnh_mlp* m, const float* params, uint64_t n);
+int nnh_mlp_get_params(nnh_mlp* m, float* params, uint64_t n);
+int nnh_mlp_forward(nnh_mlp* m, const float* x, uint64_t n, float* y);
+/* Gradient of the masked MSE (mean over unmasked targets) w.r.t. every
+ * parameter, and the loss. */
+int nnh_mlp_gradients(nnh_mlp* m, const float* x, const float* target, const float* mask, uint64_t n, float* grad,
+                      float* loss);
+/* `steps` full-batch Adam steps with learning rate lr; final loss returned. */
+int nnh_mlp_train(nnh_mlp* m, const float* x, const float* target, const float* mask, uint64_t n, int steps,
+                  float lr, float* final_loss);
+/* Index of the smallest predicted output among those with feasible > 0. */
+int nnh_mlp_choose(nnh_mlp* m, const float* x, const float* feasible, uint64_t n, int* choice);
+
 /* Device-pointer variants: operands and result already in HBM, work is
  * queued on `stream` (a hipStream_t, NULL = the library's stream) and the
  * call returns without synchronising. */

@@ -65,6 +65,16 @@ def lib():
         "nnh_mul_batch": ([ctypes.POINTER(Req), u64, vp], i32),
         "nnh_mul_algo": ([_u64p, _u64p, u64, _u64p, u64, i32, i32], i32),
         "nnh_mul_algo_device_async": ([vp, vp, u64, vp, u64, i32, i32, vp], i32),
+        "nnh_time_algo": ([vp, vp, u64, vp, u64, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_float)], i32),
+        "nnh_mlp_create": ([i32, i32, u64, ctypes.POINTER(vp)], i32),
+        "nnh_mlp_destroy": ([vp], i32),
+        "nnh_mlp_num_params": ([vp], i32),
+        "nnh_mlp_set_params": ([vp, vp, u64], i32),
+        "nnh_mlp_get_params": ([vp, vp, u64], i32),
+        "nnh_mlp_forward": ([vp, vp, u64, vp], i32),
+        "nnh_mlp_gradients": ([vp, vp, vp, vp, u64, vp, ctypes.POINTER(ctypes.c_float)], i32),
+        "nnh_mlp_train": ([vp, vp, vp, vp, u64, i32, ctypes.c_float, ctypes.POINTER(ctypes.c_float)], i32),
+        "nnh_mlp_choose": ([vp, vp, vp, u64, vp], i32),
         "nnh_device_alloc": ([ctypes.POINTER(vp), u64], i32),
         "nnh_device_free": ([vp], i32),
         "nnh_copy_to_device": ([vp, vp, u64], i32),

@@ -1,0 +1,230 @@
+"""Autotuner and learned cost model for the choice of multiplication arm.
+
+Candidates are exact arms of libnnh (any choice gives the same bits):
+schoolbook columns, one Karatsuba level, one transform for the whole
+product, the reference's one-level split, and the chunked transform at two
+piece scales.  The autotuner times every feasible arm on the GPU
+(nnh_time_algo, a C-side loop of back-to-back launches); the cost model is
+an FP32 MLP [log2 bits of the larger operand, log2 bits of the smaller,
+log2 batch count] -> predicted log2 milliseconds per arm, trained and run by
+the library's own HIP kernels (nnh_mlp_*).  The reference's own policy
+(fixed thresholds, natural.h:444-450) is arm NNH_ALGO_AUTO.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import (ALGO_AUTO, ALGO_KARATSUBA, ALGO_NTT, ALGO_NTT_CHUNK, ALGO_NTT_SPLIT, ALGO_SCHOOL, DeviceWords,
+               _check, lib)
+
+ARMS = ["school", "karatsuba", "ntt", "ntt_split", "ntt_chunk", "ntt_chunk+1"]
+N_ARMS = len(ARMS)
+N_IN = 3
+MAX_WORDS = 1 << 24
+TIME_CAP_MAC = 1 << 28     # schoolbook / Karatsuba products beyond this many word-products are not tried
+MAX_PIECES = 256           # chunked transform: at most this many pieces
+
+
+def _scale(a, b):
+    return max(0, int(a + b - 2).bit_length())
+
+
+def arm_call(arm, an, bn):
+    """(algo, param) of arm index for operand sizes (words)."""
+    big, small = max(an, bn), min(an, bn)
+    smin = max(_scale(small, small), 8)
+    return [(ALGO_SCHOOL, 0), (ALGO_KARATSUBA, 0), (ALGO_NTT, 0), (ALGO_NTT_SPLIT, 0),
+            (ALGO_NTT_CHUNK, smin), (ALGO_NTT_CHUNK, smin + 1)][arm]
+
+
+def feasible(an, bn):
+    """Arms worth timing for this shape (all arms are exact; these caps only
+    keep hopeless candidates out of the search)."""
+    big, small = max(an, bn), min(an, bn)
+    f = np.zeros(N_ARMS, dtype=np.float32)
+    f[0] = 1.0 if big * small <= TIME_CAP_MAC else 0.0
+    f[1] = 1.0 if small >= 2 and big * small <= TIME_CAP_MAC else 0.0
+    f[2] = 1.0
+    f[3] = 1.0
+    for k, arm in ((4, 4), (5, 5)):
+        sp = arm_call(arm, an, bn)[1]
+        piece = (1 << sp) - small + 1
+        pieces = -(-big // piece)
+        f[k] = 1.0 if 2 <= pieces <= MAX_PIECES else 0.0
+    return f
+
+
+def features(an, bn, count=1):
+    big, small = max(an, bn), min(an, bn)
+    return np.array([math.log2(64 * big) / 30.0, math.log2(64 * small) / 30.0, math.log2(count) / 8.0],
+                    dtype=np.float32)
+
+
+class Bench:
+    """Random operands of the largest size, resident on the device; timed
+    products point into them."""
+
+    def __init__(self, seed=0x4E4E2E19, max_words=MAX_WORDS):
+        rng = np.random.default_rng(seed)
+        self.max = max_words
+        self.a = DeviceWords(max_words, rng.integers(0, 2**64, size=max_words, dtype=np.uint64))
+        self.b = DeviceWords(max_words, rng.integers(0, 2**64, size=max_words, dtype=np.uint64))
+        self.out = DeviceWords(2 * max_words)
+
+    def time(self, algo, param, an, bn, count=1, reps=None):
+        if reps is None:
+            reps = 8 if an * bn < (1 << 22) else (3 if an + bn < (1 << 22) else 1)
+        ms = ctypes.c_float()
+        _check(lib().nnh_time_algo(self.out.ptr, self.a.ptr, an, self.b.ptr, bn, algo, param, count, reps,
+                                   ctypes.byref(ms)))
+        return float(ms.value)
+
+    def time_arm(self, arm, an, bn, count=1):
+        algo, param = arm_call(arm, an, bn)
+        return self.time(algo, param, an, bn, count)
+
+    def close(self):
+        for x in (self.a, self.b, self.out):
+            x.free()
+
+
+def grid(n_side=20, lo_bits=10, hi_bits=30, counts=(1, 8, 64)):
+    """Training grid: log-uniform (larger, smaller) operand sizes and batch counts."""
+    pts = []
+    for i in range(n_side):
+        for j in range(i + 1):
+            big = max(1, math.ceil(2 ** (lo_bits + (hi_bits - lo_bits) * i / (n_side - 1)) / 64))
+            small = max(1, math.ceil(2 ** (lo_bits + (hi_bits - lo_bits) * j / (n_side - 1)) / 64))
+            for c in counts:
+                if c * big <= (1 << 26):
+                    pts.append((big, small, c))
+    return pts
+
+
+def collect(bench, points, log=None):
+    """Measured log2(ms per product) for every feasible arm of every point."""
+    X = np.zeros((len(points), N_IN), dtype=np.float32)
+    T = np.zeros((len(points), N_ARMS), dtype=np.float32)
+    M = np.zeros((len(points), N_ARMS), dtype=np.float32)
+    for k, (an, bn, c) in enumerate(points):
+        X[k] = features(an, bn, c)
+        f = feasible(an, bn)
+        for arm in range(N_ARMS):
+            if f[arm]:
+                algo, param = arm_call(arm, an, bn)
+                ms = bench.time(algo, param, an, bn, count=c)
+                T[k, arm] = math.log2(max(ms, 1e-6))
+                M[k, arm] = 1.0
+        if log and k % 50 == 0:
+            log(f"collect {k}/{len(points)}")
+    return X, T, M
+
+
+class CostModel:
+    """The FP32 MLP in device memory (nnh_mlp_*)."""
+
+    def __init__(self, seed=1):
+        h = ctypes.c_void_p()
+        _check(lib().nnh_mlp_create(N_IN, N_ARMS, seed, ctypes.byref(h)))
+        self.h = h
+        self.n_params = lib().nnh_mlp_num_params(h)
+
+    def params(self):
+        p = np.empty(self.n_params, dtype=np.float32)
+        _check(lib().nnh_mlp_get_params(self.h, p.ctypes.data, p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().nnh_mlp_set_params(self.h, p.ctypes.data, p.size))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.empty((X.shape[0], N_ARMS), dtype=np.float32)
+        _check(lib().nnh_mlp_forward(self.h, X.ctypes.data, X.shape[0], y.ctypes.data))
+        return y
+
+    def gradients(self, X, T, M):
+        X, T, M = (np.ascontiguousarray(v, dtype=np.float32) for v in (X, T, M))
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(lib().nnh_mlp_gradients(self.h, X.ctypes.data, T.ctypes.data, M.ctypes.data, X.shape[0],
+                                       g.ctypes.data, ctypes.byref(loss)))
+        return g, float(loss.value)
+
+    def train(self, X, T, M, steps=3000, lr=3e-3):
+        X, T, M = (np.ascontiguousarray(v, dtype=np.float32) for v in (X, T, M))
+        loss = ctypes.c_float()
+        _check(lib().nnh_mlp_train(self.h, X.ctypes.data, T.ctypes.data, M.ctypes.data, X.shape[0], steps,
+                                   ctypes.c_float(lr), ctypes.byref(loss)))
+        return float(loss.value)
+
+    def choose(self, X, F):
+        X, F = (np.ascontiguousarray(v, dtype=np.float32) for v in (X, F))
+        ch = np.empty(X.shape[0], dtype=np.int32)
+        _check(lib().nnh_mlp_choose(self.h, X.ctypes.data, F.ctypes.data, X.shape[0], ch.ctypes.data))
+        return ch
+
+    def close(self):
+        if self.h:
+            lib().nnh_mlp_destroy(self.h)
+            self.h = None
+
+
+def reference_forward(p, X, n_in=N_IN, n_out=N_ARMS, H=64):
+    """Float64 NumPy restatement of the MLP (gradient checks only)."""
+    o = 0
+    W1 = p[o:o + H * n_in].reshape(H, n_in); o += H * n_in
+    b1 = p[o:o + H]; o += H
+    W2 = p[o:o + H * H].reshape(H, H); o += H * H
+    b2 = p[o:o + H]; o += H
+    W3 = p[o:o + n_out * H].reshape(n_out, H); o += n_out * H
+    b3 = p[o:o + n_out]
+    h1 = np.maximum(X @ W1.T + b1, 0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    return h2 @ W3.T + b3
+
+
+def masked_mse(p, X, T, M):
+    y = reference_forward(p, X)
+    return float(((y - T) ** 2 * M).sum() / M.sum())
+
+
+def evaluate(bench, model, requests, log=None):
+    """Exhaustive autotune of every held-out request (the judge) against the
+    model's pick and the reference policy (NNH_ALGO_AUTO)."""
+    X = np.stack([features(a, b, 1) for a, b in requests])
+    F = np.stack([feasible(a, b) for a, b in requests])
+    pick = model.choose(X, F)
+    t_best = t_model = t_auto = 0.0
+    regrets = []
+    hits = 0
+    for k, (an, bn) in enumerate(requests):
+        times = {}
+        for arm in range(N_ARMS):
+            if F[k, arm]:
+                times[arm] = bench.time_arm(arm, an, bn)
+        best = min(times.values())
+        tm = times[int(pick[k])]
+        ta = bench.time(ALGO_AUTO, 0, an, bn)
+        t_best += best
+        t_model += tm
+        t_auto += ta
+        regrets.append(tm / best - 1.0)
+        hits += tm == best
+        if log and k % 500 == 0:
+            log(f"evaluate {k}/{len(requests)}")
+    r = np.array(regrets)
+    return {
+        "requests": len(requests),
+        "total_ms_exhaustive_best": t_best,
+        "total_ms_model": t_model,
+        "total_ms_reference_policy": t_auto,
+        "regret_total": t_model / t_best - 1.0,
+        "regret_mean": float(r.mean()),
+        "regret_p90": float(np.quantile(r, 0.9)),
+        "picked_best_fraction": hits / len(requests),
+        "gain_over_reference_policy": t_auto / t_model,
+        "reference_policy_regret_total": t_auto / t_best - 1.0,
+    }

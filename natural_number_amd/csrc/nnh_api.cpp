@@ -870,6 +870,34 @@ int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an,
     return NNH_OK;
 }
 
+int nnh_time_algo(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn, int algo,
+                  int param, int count, int reps, float* ms_per_product) {
+    if (!d_dst || !d_a || !d_b || !an || !bn || count < 1 || reps < 1 || !ms_per_product)
+        return fail(NNH_ERR_ARG, "bad timing arguments");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    Device& D = g_dev;
+    hipStream_t st = D.stream;
+    int rc = mul_algo_device(D, d_dst, d_a, an, d_b, bn, algo, param, st);  // warm-up, grows buffers
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps && !rc; r++)
+        for (int c = 0; c < count && !rc; c++) rc = mul_algo_device(D, d_dst, d_a, an, d_b, bn, algo, param, st);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *ms_per_product = ms / (float)(reps * count);
+    return NNH_OK;
+}
+
 int nnh_mul_algo(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn, int algo, int param) {
     if (!dst || !a || !b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
     NEED_INIT();

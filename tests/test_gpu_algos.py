@@ -1,0 +1,69 @@
+"""Every explicit algorithm arm is exact (against the oracle), including the
+Karatsuba level's 65-bit coefficient carries and many-piece chunked
+transforms; and the cost model's GPU gradients against float64 finite
+differences."""
+import numpy as np
+import pytest
+
+from conftest import rand_limbs
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(1, 1), (2, 2), (5, 3), (100, 100), (383, 1000), (1000, 1000), (3000, 50), (5000, 4000),
+         (70000, 3000), (40000, 40001)]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("algo", [1, 2, 3, 4, 5])
+def test_arms_exact(nnh, oracle, algo):
+    rng = np.random.default_rng(algo)
+    for an, bn in SIZES:
+        a, b = rand_limbs(rng, an), rand_limbs(rng, bn)
+        say(f"algo {nnh.ALGO_NAMES[algo]} {an}x{bn}")
+        assert np.array_equal(nnh.mul_algo(a, b, algo), oracle.mul(a, b)), (an, bn)
+
+
+def test_karatsuba_all_ones(nnh, oracle):
+    for n, m in [(4096, 4096), (3001, 1999), (2, 2)]:
+        a = np.full(n, 2**64 - 1, dtype=np.uint64)
+        b = np.full(m, 2**64 - 1, dtype=np.uint64)
+        say(f"karatsuba all-ones {n}x{m}")
+        assert np.array_equal(nnh.mul_algo(a, b, nnh.ALGO_KARATSUBA), oracle.mul(a, b))
+
+
+@pytest.mark.parametrize("sp", [9, 10, 12, 13])
+def test_chunked_piece_scales(nnh, oracle, sp):
+    rng = np.random.default_rng(sp)
+    a, b = rand_limbs(rng, 100000), rand_limbs(rng, 300)
+    say(f"chunked sp={sp}")
+    assert np.array_equal(nnh.mul_algo(a, b, nnh.ALGO_NTT_CHUNK, sp), oracle.mul(a, b))
+
+
+def test_mlp_gradients_vs_finite_differences(nnh):
+    from natural_number_amd import costmodel as cm
+
+    rng = np.random.default_rng(0)
+    model = cm.CostModel(seed=3)
+    n = 48
+    X = rng.uniform(0.3, 1.0, size=(n, cm.N_IN)).astype(np.float32)
+    T = rng.normal(0, 2, size=(n, cm.N_ARMS)).astype(np.float32)
+    M = (rng.uniform(size=(n, cm.N_ARMS)) < 0.7).astype(np.float32)
+    g, loss = model.gradients(X, T, M)
+    p = model.params().astype(np.float64)
+    Xd, Td, Md = X.astype(np.float64), T.astype(np.float64), M.astype(np.float64)
+    assert abs(loss - cm.masked_mse(p, Xd, Td, Md)) < 1e-4 * max(1.0, loss)
+    idx = rng.choice(p.size, 60, replace=False)
+    h = 1e-5
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (cm.masked_mse(pp, Xd, Td, Md) - cm.masked_mse(pm, Xd, Td, Md)) / (2 * h)
+        assert abs(g[i] - fd) <= 2e-3 * max(1.0, abs(fd)), (i, g[i], fd)
+    # training lowers the loss
+    l1 = model.train(X, T, M, steps=300, lr=3e-3)
+    assert l1 < loss
+    model.close()
