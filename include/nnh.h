@@ -32,7 +32,8 @@ typedef enum nnh_status {
     NNH_ERR_ARG = 1,       /* null pointer, zero length, size out of range */
     NNH_ERR_NOMEM = 2,     /* device allocation failed */
     NNH_ERR_HIP = 3,       /* a HIP runtime call failed */
-    NNH_ERR_NODEV = 4      /* no usable device / nnh_init not called */
+    NNH_ERR_NODEV = 4,     /* no usable device / nnh_init not called */
+    NNH_ERR_DOMAIN = 5     /* input outside the reference algorithm's domain */
 } nnh_status;
 
 /* Select and initialise a device (builds the root-base constants).
@@ -180,6 +181,34 @@ int nnh_event_elapsed_ms(void* start, void* stop, float* ms);
 int nnh_event_destroy(void* ev);
 
 /* Human-readable text of the last error on this thread. */
+/* ---- Device-resident natural layer ------------------------------------------
+ * Host buffers in and out; limbs stay in device memory in between and every
+ * product, addition, shift and comparison runs on the GPU.  Outputs are
+ * normalised (top word nonzero unless the value is 0). */
+
+/* q = a / b, r = a % b: the reference natural::_div (natural.h:749-769) with
+ * its Newton reciprocal and div_iterative (natural.h:566-747), including the
+ * cyclic products of NTT_info::save (ntt_info.h:237-248).  Returns
+ * NNH_ERR_DOMAIN for divisors outside the reference reciprocal's domain
+ * (natural.h:675-677 would write ~2^64 words). */
+int nnh_divmod(uint64_t* q, uint64_t q_cap, uint64_t* qn, uint64_t* r, uint64_t r_cap, uint64_t* rn,
+               const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn);
+/* b^e by the reference's square-and-multiply recursion (pow, natural.h:554-564). */
+int nnh_pow(uint64_t* dst, uint64_t cap, uint64_t* n, const uint64_t* b, uint64_t bn, uint64_t e);
+/* Decimal digits of a (operator<< with std::dec, natural.h:1020-1039 via
+ * dec_init/_print_dec/_dec_split, natural.h:904-992); out gets a
+ * NUL-terminated string of *len digits. */
+int nnh_to_dec(char* out, uint64_t cap, uint64_t* len, const uint64_t* a, uint64_t an);
+/* calc_e(num, den, 0, 2^k) (natural.h:841-857), level-synchronous. */
+int nnh_calc_e(uint64_t* num, uint64_t num_cap, uint64_t* num_n, uint64_t* den, uint64_t den_cap, uint64_t* den_n,
+               int k);
+/* The reference's e recipe (test.cpp:12-19): calc_e over [0, 2^k), times
+ * 10^digits, divided by den, printed in decimal.  stage_ms (NNH_E_STAGES
+ * entries, may be NULL) receives the wall time of calc_e, pow, the
+ * product, the division and the decimal output. */
+#define NNH_E_STAGES 5
+int nnh_e_digits(char* out, uint64_t cap, uint64_t* len, int k, uint64_t digits, double* stage_ms);
+
 const char* nnh_last_error(void);
 
 #ifdef __cplusplus

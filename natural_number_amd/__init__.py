@@ -87,6 +87,11 @@ def lib():
         "nnh_event_record": ([vp, vp], i32),
         "nnh_event_elapsed_ms": ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
         "nnh_event_destroy": ([vp], i32),
+        "nnh_divmod": ([_u64p, u64, _u64p, _u64p, u64, _u64p, _u64p, u64, _u64p, u64], i32),
+        "nnh_pow": ([_u64p, u64, _u64p, _u64p, u64, u64], i32),
+        "nnh_to_dec": ([ctypes.c_char_p, u64, _u64p, _u64p, u64], i32),
+        "nnh_calc_e": ([_u64p, u64, _u64p, _u64p, u64, _u64p, i32], i32),
+        "nnh_e_digits": ([ctypes.c_char_p, u64, _u64p, i32, u64, ctypes.POINTER(ctypes.c_double)], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -300,6 +305,64 @@ class Event:
 
 def synchronize(stream=None):
     _check(lib().nnh_synchronize(stream.ptr if isinstance(stream, Stream) else stream))
+
+
+# ---- device-resident natural layer -------------------------------------------
+def divmod_(a, b):
+    """(a // b, a % b) by the reference's Newton division on the GPU
+    (natural.h:749-769); both normalised uint64 limb arrays."""
+    a, b = _limbs(a), _limbs(b)
+    q = np.zeros(a.size + 1, dtype=np.uint64)
+    r = np.zeros(b.size + 1, dtype=np.uint64)
+    qn, rn = ctypes.c_uint64(), ctypes.c_uint64()
+    _check(lib().nnh_divmod(_arr(q), q.size, ctypes.byref(qn), _arr(r), r.size, ctypes.byref(rn), _arr(a), a.size,
+                            _arr(b), b.size))
+    return q[:qn.value].copy(), r[:rn.value].copy()
+
+
+def pow_(b, e):
+    """b^e (natural.h:554-564), normalised limbs."""
+    b = _limbs(b)
+    bits = max(1, int(np.uint64(b[-1])).bit_length() + 64 * (b.size - 1))
+    cap = (bits * max(e, 1)) // 64 + 2
+    out = np.zeros(cap, dtype=np.uint64)
+    n = ctypes.c_uint64()
+    _check(lib().nnh_pow(_arr(out), cap, ctypes.byref(n), _arr(b), b.size, e))
+    return out[:n.value].copy()
+
+
+def to_dec(a):
+    """Decimal string of a (natural.h:1020-1039)."""
+    a = _limbs(a)
+    cap = int(a.size * 64 * 0.30103) + 64
+    buf = ctypes.create_string_buffer(cap)
+    n = ctypes.c_uint64()
+    _check(lib().nnh_to_dec(buf, cap, ctypes.byref(n), _arr(a), a.size))
+    return buf.raw[:n.value].decode()
+
+
+def calc_e(k):
+    """(num, den) of calc_e(0, 2^k) (natural.h:841-857)."""
+    cap = (24 << k) // 64 + 64 if k <= 24 else (k << k) // 64 + 64
+    num = np.zeros(cap, dtype=np.uint64)
+    den = np.zeros(cap, dtype=np.uint64)
+    nn, dn = ctypes.c_uint64(), ctypes.c_uint64()
+    _check(lib().nnh_calc_e(_arr(num), cap, ctypes.byref(nn), _arr(den), cap, ctypes.byref(dn), k))
+    return num[:nn.value].copy(), den[:dn.value].copy()
+
+
+E_STAGES = ["calc_e", "pow", "mul", "div", "to_dec"]
+
+
+def e_digits(k, digits):
+    """The reference's e recipe (test.cpp:12-19): returns (digit string,
+    {stage: ms})."""
+    cap = digits + 64
+    buf = ctypes.create_string_buffer(cap)
+    n = ctypes.c_uint64()
+    ms = (ctypes.c_double * len(E_STAGES))()
+    _check(lib().nnh_e_digits(buf, cap, ctypes.byref(n), k, digits, ms))
+    return buf.raw[:n.value].decode(), dict(zip(E_STAGES, list(ms)))
 
 
 def threshold_words():

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/nnh.h"
+#include "nnh_host.hpp"
 #include "nnh_internal.hpp"
 
 namespace nnh {
@@ -50,6 +51,7 @@ static int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
 #define HIPCHK(x)                                                                  \
     do {                                                                           \
         hipError_t e_ = (x);                                                       \
@@ -650,7 +652,7 @@ static void batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
     launch_crt_batch(B->d_jobs, (u32)B->jobs.size(), B->ntiles, B->d_tiles, B->d_tiles + B->ntiles, D.crt, st);
 }
 
-static int batch_create(Device& D, const nnh_req* reqs, u64 n, int flags, nnh_batch** out) {
+static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch** out) {
     nnh_batch* B = new nnh_batch();
     struct Item {
         u64 idx;
@@ -659,10 +661,11 @@ static int batch_create(Device& D, const nnh_req* reqs, u64 n, int flags, nnh_ba
     std::vector<Item> items;
     items.reserve(n);
     for (u64 i = 0; i < n; i++) {
-        const nnh_req& r = reqs[i];
-        if (!r.dst || !r.a || !r.b || !r.an || !r.bn) {
+        const IReq& r = reqs[i];
+        if (!r.dst || !r.a || !r.b || !r.an || !r.bn || !r.n_out || r.n_out > r.an + r.bn + 1 ||
+            (r.n_add && !r.add) || r.n_add > r.n_out) {
             delete B;
-            return fail(NNH_ERR_ARG, "batch request " + std::to_string(i) + ": null pointer or empty operand");
+            return fail(NNH_ERR_ARG, "batch request " + std::to_string(i) + ": null pointer or bad size");
         }
         u64 mn = std::min(r.an, r.bn);
         int s = mn < NTT_MUL_THRESHOLD ? 0 : ntt_scale(r.an, r.bn);
@@ -677,7 +680,7 @@ static int batch_create(Device& D, const nnh_req* reqs, u64 n, int flags, nnh_ba
     // workspace offsets
     u64 ws = 0, tiles = 0, cols = 0;
     for (const Item& it : items) {
-        const nnh_req& r = reqs[it.idx];
+        const IReq& r = reqs[it.idx];
         BJob J{};
         bool sq = r.a == r.b && r.an == r.bn;
         J.a = r.a;
@@ -685,7 +688,9 @@ static int batch_create(Device& D, const nnh_req* reqs, u64 n, int flags, nnh_ba
         J.an = r.an;
         J.bn = r.bn;
         J.out = r.dst;
-        J.n_out = r.an + r.bn;
+        J.n_out = r.n_out;
+        J.add = r.add;
+        J.n_add = r.n_add;
         J.tile0 = tiles;
         tiles += crt_tiles(J.n_out);
         if (it.s == 0) {
@@ -750,6 +755,52 @@ static int batch_create(Device& D, const nnh_req* reqs, u64 n, int flags, nnh_ba
         HIPCHK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
     }
     *out = B;
+    return NNH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Entry points for the device-resident natural layer (nnh_host.hpp).
+// ---------------------------------------------------------------------------
+int host_init() { return g_inited ? NNH_OK : device_init(0); }
+hipStream_t host_stream() { return g_dev.stream; }
+std::mutex& host_mutex() { return g_dev.mu; }
+
+int dev_mul(u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st) {
+    return mul_device(g_dev, out, a, an, b, bn, st);
+}
+int dev_sqr(u64* out, const u64* a, u64 an, hipStream_t st) { return sqr_device(g_dev, out, a, an, st); }
+
+int dev_mul_scale(u64* out, const u64* a, u64 an, const u64* b, u64 bn, int s, u64 save_size, u64* n_out,
+                  hipStream_t st) {
+    Device& D = g_dev;
+    u64 L = 1ull << s;
+    if (s < MIN_MID_BITS || s > MAX_SCALE || an > L || bn > L) return fail(NNH_ERR_ARG, "bad transform scale");
+    if (save_size - 1 <= L) {
+        *n_out = an + bn;
+        return mul_device(D, out, a, an, b, bn, st);
+    }
+    Plan pl = make_plan(D, s);
+    int rc;
+    if ((rc = ensure_twiddles(D, s)) || (rc = ensure(D.ws, 6 * pl.L))) return rc;
+    if ((rc = product_core(D, pl, a, an, b, bn, false, nullptr, out, L + 2, nullptr, 0, st))) return rc;
+    {
+        ProfScope ps(K_FOLD, 16 * L, st);
+        launch_cyclic_fold(out, L, st);
+    }
+    *n_out = L;
+    return NNH_OK;
+}
+
+int dev_batch(const IReq* reqs, u64 n, hipStream_t st) {
+    if (!n) return NNH_OK;
+    nnh_batch* B = nullptr;
+    int rc = batch_create(g_dev, reqs, n, 0, &B);
+    if (rc) return rc;
+    batch_enqueue(g_dev, B, st);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    batch_free(B);
+    if (e != hipSuccess) return fail(NNH_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
     return NNH_OK;
 }
 
@@ -1011,7 +1062,10 @@ int nnh_batch_create(const nnh_req* reqs, uint64_t n, int flags, nnh_batch** out
     NEED_INIT();
     std::lock_guard<std::mutex> g(g_dev.mu);
     *out = nullptr;
-    return batch_create(g_dev, reqs, n, flags, out);
+    std::vector<IReq> ir(n);
+    for (u64 i = 0; i < n; i++)
+        ir[i] = IReq{reqs[i].dst, reqs[i].a, reqs[i].an, reqs[i].b, reqs[i].bn, reqs[i].an + reqs[i].bn, nullptr, 0};
+    return batch_create(g_dev, ir.data(), n, flags, out);
 }
 
 int nnh_batch_run(nnh_batch* b, void* stream) {

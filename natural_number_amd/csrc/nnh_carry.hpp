@@ -1,0 +1,171 @@
+// nnh_carry.hpp — carry machinery shared by the CRT, the limb arithmetic
+// and the small-product kernels: Garner reconstruction with a wave-level
+// carry-lookahead over 4096-word tiles, the single work-group scan of tile
+// carry functions and the carry fix-up.
+#pragma once
+#include "nnh_arith.hpp"
+#include "nnh_internal.hpp"
+
+namespace nnh {
+
+// CRT + carry.  Garner (p0, p2, then p1 as in reference asmCRT,
+// asmLinux.asm:1369-1450) gives X_k < 2^186 as three words; the product
+// words are  sum_k X_k B^k (+ an optional addend).  With
+//   D_k = x0_k + x1_{k-1} + x2_{k-2} + add_k = dlo_k + 2^64 dhi_k,
+//   E_k = dlo_k + dhi_{k-1} <= 2^64 + 2,
+// every position generates (E_k >= 2^64) or propagates (E_k == 2^64-1) a
+// single carry bit; a wave resolves 64 of them with one 64-bit add of its
+// ballots (carry-ins = (A + G + cin) ^ P, A = G|P).  Tiles resolve with
+// carry-in 0 and publish their carry function; a scan and a fix-up pass
+// apply the tile carry-ins.
+// ---------------------------------------------------------------------------
+struct CrtArgs {
+    const u64* r[3];      // residues, or the three words of column sums
+    int columns;          // 1: r[] already holds X_k as three words
+    u64 n_res;            // positions carrying residues
+    u64* out;
+    u64 n_out;            // output words
+    const u64* addend;    // optional, n_add words added at position 0
+    u64 n_add;
+    u8* tile_fn;          // per tile: bit0 = carry-out with cin 0, bit1 = with cin 1
+    CrtConsts C;
+};
+
+constexpr int CRT_TILE_ROUNDS = 16;  // 16 x 256 positions per tile
+
+__device__ __forceinline__ void crt_x(const CrtArgs& A, u64 k, u64& x0, u64& x1, u64& x2) {
+    if (k >= A.n_res) x0 = x1 = x2 = 0;
+    else if (A.columns) { x0 = A.r[0][k]; x1 = A.r[1][k]; x2 = A.r[2][k]; }
+    else garner(A.r[0][k], A.r[1][k], A.r[2][k], A.C.c02m, A.C.m0m, A.C.c1m, A.C.m02lo, A.C.m02hi, x0, x1, x2);
+}
+
+__device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn_slot, u64 (*xs)[256 + 3],
+                                              u32 (*wco)[4]) {
+    const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const u64 tile_base = tile * (256 * CRT_TILE_ROUNDS);
+    __syncthreads();  // LDS reuse across persistent iterations
+    if (t < 3) {
+        u64 x0 = 0, x1 = 0, x2 = 0;
+        if (tile_base >= 3 - t) crt_x(A, tile_base - 3 + t, x0, x1, x2);
+        xs[0][t] = x0; xs[1][t] = x1; xs[2][t] = x2;
+    }
+    u32 cin0 = 0, cin1 = 1;  // running carry assuming tile carry-in 0 / 1
+    for (int r = 0; r < CRT_TILE_ROUNDS; r++) {
+        const u64 k = tile_base + (u64)r * 256 + t;
+        if (k >= A.n_out && tile_base + (u64)r * 256 >= A.n_out) break;  // uniform
+        u64 x0, x1, x2;
+        crt_x(A, k, x0, x1, x2);
+        __syncthreads();
+        xs[0][3 + t] = x0; xs[1][3 + t] = x1; xs[2][3 + t] = x2;
+        __syncthreads();
+        // D_k and D_{k-1}
+        u64 ad = (A.addend && k < A.n_add) ? A.addend[k] : 0;
+        u64 a1 = xs[1][2 + t], a2 = xs[2][1 + t];
+        // low word of D_k; its high part reaches position k+1, which
+        // recomputes it from the same LDS words (cp below)
+        u64 dlo = x0 + a1 + a2 + ad;
+        u64 ad1 = (A.addend && k >= 1 && k - 1 < A.n_add) ? A.addend[k - 1] : 0;
+        u64 p0 = xs[0][2 + t], p1 = xs[1][1 + t], p2 = xs[2][t];
+        u64 q = p0 + p1;
+        u32 cp = q < p0 ? 1u : 0u;
+        u64 q2 = q + p2;
+        cp += q2 < q ? 1u : 0u;
+        u64 q3 = q2 + ad1;
+        cp += q3 < q2 ? 1u : 0u;
+        u64 e = dlo + cp;
+        bool g = e < dlo;
+        bool pp = e == ~0ull;
+        u64 G = __ballot(g), P = __ballot(pp), Am = G | P;
+        // wave carry functions
+        u64 s0 = Am + G;
+        u32 co0 = s0 < Am ? 1u : 0u;
+        u64 s1 = s0 + 1;
+        u32 co1 = co0 | (s1 == 0 ? 1u : 0u);
+        if (lane == 0) { wco[0][wv] = co0; wco[1][wv] = co1; }
+        __syncthreads();
+        u32 c0 = cin0, c1 = cin1;
+        for (u32 w = 0; w < wv; w++) {
+            c0 = c0 ? wco[1][w] : wco[0][w];
+            c1 = c1 ? wco[1][w] : wco[0][w];
+        }
+        u64 sw = c0 ? s1 : s0;
+        u32 cin_lane = (u32)(((sw ^ P) >> lane) & 1u);
+        if (k < A.n_out) A.out[k] = e + cin_lane;
+        // round carry-out for the next round
+        u32 n0 = cin0, n1 = cin1;
+        for (u32 w = 0; w < 4; w++) {
+            n0 = n0 ? wco[1][w] : wco[0][w];
+            n1 = n1 ? wco[1][w] : wco[0][w];
+        }
+        cin0 = n0; cin1 = n1;
+        __syncthreads();
+        if (t >= 253) {
+            xs[0][t - 253] = x0; xs[1][t - 253] = x1; xs[2][t - 253] = x2;
+        }
+    }
+    if (t == 0) *fn_slot = (u8)(cin0 | (cin1 << 1));
+}
+
+
+// Single work-group scan of tile carry functions -> carry-in per tile.
+// Each tile function is kill (00), generate (11) or propagate (10).  cin0 is
+// the carry into the first tile; the carry out of the last is stored to
+// *cout when cout is non-null.
+__device__ __forceinline__ void carry_scan_body(const u8* tile_fn, u64 n_tiles, u8* tile_cin, u32* chunk_fn,
+                                                u32 cin0 = 0, u32* cout = nullptr) {
+    const u32 t = threadIdx.x;
+    const u64 per = (n_tiles + 1023) / 1024;
+    const u64 b0 = (u64)t * per, b1 = b0 + per < n_tiles ? b0 + per : n_tiles;
+    u32 f0 = 0, f1 = 1;  // composed function of this thread's tiles
+    for (u64 i = b0; i < b1; i++) {
+        u32 f = tile_fn[i];
+        u32 o0 = f & 1u, o1 = (f >> 1) & 1u;
+        f0 = f0 ? o1 : o0;
+        f1 = f1 ? o1 : o0;
+    }
+    chunk_fn[t] = f0 | (f1 << 1);
+    __syncthreads();
+    // serial exclusive scan over 1024 thread functions (cheap: 1024 steps)
+    if (t == 0) {
+        u32 c = cin0;
+        for (u32 i = 0; i < 1024; i++) {
+            u32 f = chunk_fn[i];
+            chunk_fn[i] = c;
+            c = c ? (f >> 1) & 1u : f & 1u;
+        }
+        if (cout) *cout = c;
+    }
+    __syncthreads();
+    u32 c = chunk_fn[t];
+    for (u64 i = b0; i < b1; i++) {
+        tile_cin[i] = (u8)c;
+        u32 f = tile_fn[i];
+        c = c ? (f >> 1) & 1u : f & 1u;
+    }
+}
+
+// Add the tile carry-in (0/1) at the tile's first word and propagate it
+// through the tile's all-ones words.
+__device__ __forceinline__ void carry_fix_body(u64* out, u64 n_out, u64 tile, u32& first_nz) {
+    const u32 t = threadIdx.x;
+    const u64 tile_base = tile * (256 * CRT_TILE_ROUNDS);
+    for (int r = 0; r < CRT_TILE_ROUNDS; r++) {
+        u64 base = tile_base + (u64)r * 256;
+        if (base >= n_out) return;
+        u64 k = base + t;
+        u64 w = k < n_out ? out[k] : 0;
+        if (t == 0) first_nz = 256;
+        __syncthreads();
+        if (k < n_out && w != ~0ull) atomicMin(&first_nz, t);
+        __syncthreads();
+        u32 f = first_nz;
+        if (k < n_out) {
+            if (t < f) out[k] = 0;
+            else if (t == f) out[k] = w + 1;
+        }
+        if (f < 256) return;
+        __syncthreads();
+    }
+}
+
+}  // namespace nnh

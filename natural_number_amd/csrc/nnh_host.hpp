@@ -1,0 +1,45 @@
+// nnh_host.hpp — host-side entry points of the product engine used by the
+// device-resident natural layer (nnh_natural.cpp).  Pointers are device
+// pointers; work is queued on st.  Callers hold host_mutex().
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "nnh_internal.hpp"
+
+namespace nnh {
+
+int host_init();                  // initialise device 0 if nnh_init was not called
+hipStream_t host_stream();
+std::mutex& host_mutex();
+int set_error(int code, const std::string& msg);
+
+// out[0 .. an+bn) = a * b, the reference dispatch (natural.h:444-450).
+int dev_mul(u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st);
+// out[0 .. 2 an) = a^2 (natural.h:529-535).
+int dev_sqr(u64* out, const u64* a, u64 an, hipStream_t st);
+// a * b through a transform of 2^s words with the reference's save rule
+// (ntt_info.h:237-248): when save_size - 1 > 2^s the product modulo
+// B^(2^s) - 1 (2^s words), otherwise the exact product (save_size = an+bn
+// words).  an, bn <= 2^s.  out holds max(an+bn, 2^s + 2) words.
+int dev_mul_scale(u64* out, const u64* a, u64 an, const u64* b, u64 bn, int s, u64 save_size, u64* n_out,
+                  hipStream_t st);
+
+// One request of an internal batch: dst[0 .. n_out) = a * b + add (the
+// value must fit n_out words; n_out <= an + bn).
+struct IReq {
+    u64* dst;
+    const u64* a;
+    u64 an;
+    const u64* b;
+    u64 bn;
+    u64 n_out;
+    const u64* add;
+    u64 n_add;
+};
+// Plan, enqueue, synchronise and release one batch.
+int dev_batch(const IReq* reqs, u64 n, hipStream_t st);
+
+}  // namespace nnh

@@ -78,6 +78,8 @@ struct BJob {
     u64 n_out;
     u64 tile0;        // first global CRT tile of this job
     u64 col0;         // first global school column (school jobs)
+    const u64* add;   // optional addend (n_add words) summed into out
+    u64 n_add;
     u32 school;       // 1: schoolbook columns
     u32 sqr;          // 1: b == a (single transform)
 };

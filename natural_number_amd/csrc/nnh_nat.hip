@@ -1,0 +1,552 @@
+// nnh_nat.hip — limb arithmetic on device-resident naturals: batched
+// modular add/sub with tile carry functions, shifts, complement, searches,
+// and the kernels of the binary-splitting and radix-conversion workloads
+// (reference natural.h:841-857 and 904-1039).
+#include "nnh_carry.hpp"
+#include "nnh_nat.hpp"
+#include "nnh_tile.hpp"
+
+namespace nnh {
+
+// ---------------------------------------------------------------------------
+// Batched modular addition.  Tiles of 4096 words resolve their carries with
+// wave ballots (the same lookahead as the CRT: generate = overflow,
+// propagate = all-ones word) assuming carry-in 0 and publish their carry
+// function; one scan per job applies the job's carry-in and one fix-up pass
+// adds the tile carry-ins.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u32 find_add_job(const AddJob* jobs, u32 njobs, u64 tile) {
+    u32 lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        u32 mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_add_tiles(const AddJob* __restrict__ jobs, u32 njobs, u64 ntiles,
+                                                   u8* tile_fn) {
+    __shared__ u32 wco[2][4];
+    const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (u64 it = blockIdx.x; it < ntiles; it += gridDim.x) {
+        const AddJob J = jobs[find_add_job(jobs, njobs, it)];
+        const u64 tile_base = (it - J.tile0) * LIMB_TILE;
+        u32 cin0 = 0, cin1 = 1;
+        for (int r = 0; r < CRT_TILE_ROUNDS; r++) {
+            const u64 base = tile_base + (u64)r * 256;
+            if (base >= J.n) break;  // uniform
+            const u64 k = base + t;
+            u64 x = k < J.na ? J.a[k] : 0;
+            u64 y = k < J.nb ? J.b[k] : 0;
+            if (J.sub) y = ~y;
+            u64 e = x + y;
+            // positions past n pass the carry through, so the tile's carry
+            // out is the carry out of word n-1
+            bool g = k < J.n && e < x;
+            bool pp = k >= J.n || e == ~0ull;
+            u64 G = __ballot(g), P = __ballot(pp), Am = G | P;
+            u64 s0 = Am + G;
+            u32 co0 = s0 < Am ? 1u : 0u;
+            u64 s1 = s0 + 1;
+            u32 co1 = co0 | (s1 == 0 ? 1u : 0u);
+            __syncthreads();  // previous round's readers of wco are done
+            if (lane == 0) { wco[0][wv] = co0; wco[1][wv] = co1; }
+            __syncthreads();
+            u32 c0 = cin0;
+            for (u32 w = 0; w < wv; w++) c0 = c0 ? wco[1][w] : wco[0][w];
+            u64 sw = c0 ? s1 : s0;
+            u32 cl = (u32)(((sw ^ P) >> lane) & 1u);
+            if (k < J.n) J.out[k] = e + cl;
+            u32 n0 = cin0, n1 = cin1;
+            for (u32 w = 0; w < 4; w++) {
+                n0 = n0 ? wco[1][w] : wco[0][w];
+                n1 = n1 ? wco[1][w] : wco[0][w];
+            }
+            cin0 = n0;
+            cin1 = n1;
+        }
+        if (t == 0) tile_fn[it] = (u8)(cin0 | (cin1 << 1));
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_add_scan(const AddJob* __restrict__ jobs, u32 njobs, u64 ntiles,
+                                                   const u8* tile_fn, u8* tile_cin) {
+    __shared__ u32 chunk_fn[1024];
+    for (u32 j = blockIdx.x; j < njobs; j += gridDim.x) {
+        const AddJob& J = jobs[j];
+        const u64 t0 = J.tile0, t1 = j + 1 < njobs ? jobs[j + 1].tile0 : ntiles;
+        const u32 cin = J.sub ? 1u : (J.cin_ptr ? *J.cin_ptr : J.cin);
+        __syncthreads();
+        carry_scan_body(tile_fn + t0, t1 - t0, tile_cin + t0, chunk_fn, cin, J.cout);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_add_fix(const AddJob* __restrict__ jobs, u32 njobs, u64 ntiles,
+                                                 const u8* tile_cin) {
+    __shared__ u32 first_nz;
+    for (u64 it = blockIdx.x; it < ntiles; it += gridDim.x) {
+        if (!tile_cin[it]) continue;  // uniform
+        const AddJob& J = jobs[find_add_job(jobs, njobs, it)];
+        __syncthreads();
+        carry_fix_body(J.out, J.n, it - J.tile0, first_nz);
+    }
+}
+
+u64 add_tiles_needed(const AddJob* jobs, u64 njobs) {
+    u64 t = 0;
+    for (u64 j = 0; j < njobs; j++) t += (jobs[j].n + LIMB_TILE - 1) / LIMB_TILE;
+    return t;
+}
+
+void launch_add_batch(AddJob* d_jobs, u32 njobs, u64 ntiles, u8* d_tiles, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_tiles, dim3(persist_grid(ntiles, 8192)), dim3(256), 0, st, (const AddJob*)d_jobs, njobs,
+                       ntiles, d_tiles);
+    hipLaunchKernelGGL(k_add_scan, dim3(persist_grid(njobs, 1024)), dim3(1024), 0, st, (const AddJob*)d_jobs, njobs,
+                       ntiles, (const u8*)d_tiles, d_tiles + ntiles);
+    hipLaunchKernelGGL(k_add_fix, dim3(persist_grid(ntiles, 8192)), dim3(256), 0, st, (const AddJob*)d_jobs, njobs,
+                       ntiles, (const u8*)(d_tiles + ntiles));
+}
+
+// ---------------------------------------------------------------------------
+// Word-parallel helpers
+// ---------------------------------------------------------------------------
+__global__ void k_not(u64* out, const u64* a, u64 n) {
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) out[k] = ~a[k];
+}
+__global__ void k_fill(u64* out, u64 n, u64 v) {
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) out[k] = v;
+}
+__global__ void k_shl_bits(u64* out, const u64* a, u64 n, u32 b) {
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k <= n; k += (u64)gridDim.x * blockDim.x) {
+        u64 lo = k < n ? a[k] << b : 0;
+        u64 hi = k > 0 ? a[k - 1] >> (64 - b) : 0;
+        out[k] = lo | hi;
+    }
+}
+__global__ void k_shr_bits(u64* out, const u64* a, u64 n, u32 b) {
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) {
+        u64 lo = a[k] >> b;
+        u64 hi = k + 1 < n ? a[k + 1] << (64 - b) : 0;
+        out[k] = lo | hi;
+    }
+}
+__global__ void k_set_word(u64* p, u64 v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+__global__ __launch_bounds__(256) void k_find(const u64* a, const u64* b, u64 n, int mode, u64* res3) {
+    unsigned long long best = 0;
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) {
+        u64 x = a[k];
+        bool hit = mode == 0 ? x != 0 : (mode == 1 ? x != ~0ull : x != b[k]);
+        if (hit) best = k + 1;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(best, off);
+        best = o > best ? o : best;
+    }
+    if ((threadIdx.x & 63) == 0 && best) atomicMax((unsigned long long*)res3, best);
+}
+__global__ void k_fetch(u64* res3, const u64* a, const u64* b) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    u64 i = res3[0];
+    res3[1] = i ? a[i - 1] : 0;
+    res3[2] = (i && b) ? b[i - 1] : 0;
+}
+
+static unsigned grid_for(u64 n) { return persist_grid((n + 255) / 256, 4096); }
+
+void launch_not(u64* out, const u64* a, u64 n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_not, dim3(grid_for(n)), dim3(256), 0, st, out, a, n);
+}
+void launch_fill(u64* out, u64 n, u64 v, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, st, out, n, v);
+}
+void launch_shl_bits(u64* out, const u64* a, u64 n, u32 b, hipStream_t st) {
+    hipLaunchKernelGGL(k_shl_bits, dim3(grid_for(n + 1)), dim3(256), 0, st, out, a, n, b);
+}
+void launch_shr_bits(u64* out, const u64* a, u64 n, u32 b, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_shr_bits, dim3(grid_for(n)), dim3(256), 0, st, out, a, n, b);
+}
+void launch_set_word(u64* p, u64 v, hipStream_t st) { hipLaunchKernelGGL(k_set_word, dim3(1), dim3(64), 0, st, p, v); }
+void launch_find(const u64* a, const u64* b, u64 n, int mode, u64* res3, hipStream_t st) {
+    launch_set_word(res3, 0, st);
+    if (n) hipLaunchKernelGGL(k_find, dim3(persist_grid((n + 255) / 256, 2048)), dim3(256), 0, st, a, b, n, mode, res3);
+    hipLaunchKernelGGL(k_fetch, dim3(1), dim3(64), 0, st, res3, a, b);
+}
+
+// ---------------------------------------------------------------------------
+// Work-group multiword helpers (256 threads; operands in LDS or global).
+// ---------------------------------------------------------------------------
+// x[0..n) = x + (sub ? ~y : y) + cin, y zero-extended beyond ny; returns the
+// carry out of word n-1 (uniform).
+__device__ u32 wg_addsub(u64* x, u64 n, const u64* y, u64 ny, u32 sub, u32 cin, u32 (*wco)[4]) {
+    const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    u32 c = cin;
+    for (u64 base = 0; base < n; base += 256) {
+        const u64 k = base + t;
+        u64 xv = k < n ? x[k] : 0;
+        u64 yv = k < ny ? y[k] : 0;
+        if (sub) yv = ~yv;
+        u64 e = xv + yv;
+        bool g = k < n && e < xv;
+        bool pp = k >= n || e == ~0ull;  // past the end: carry passes through
+        u64 G = __ballot(g), P = __ballot(pp), Am = G | P;
+        u64 s0 = Am + G;
+        u32 co0 = s0 < Am ? 1u : 0u;
+        u64 s1 = s0 + 1;
+        u32 co1 = co0 | (s1 == 0 ? 1u : 0u);
+        __syncthreads();
+        if (lane == 0) { wco[0][wv] = co0; wco[1][wv] = co1; }
+        __syncthreads();
+        u32 cw = c;
+        for (u32 w = 0; w < wv; w++) cw = cw ? wco[1][w] : wco[0][w];
+        u64 sw = cw ? s1 : s0;
+        u32 cl = (u32)(((sw ^ P) >> lane) & 1u);
+        if (k < n) x[k] = e + cl;
+        for (u32 w = 0; w < 4; w++) c = c ? wco[1][w] : wco[0][w];
+    }
+    __syncthreads();
+    return c;
+}
+
+// Sign of x - y (x: nx words, y: ny words, both zero-extended); uniform.
+__device__ int wg_cmp(const u64* x, u64 nx, const u64* y, u64 ny, u32* sh) {
+    const u32 t = threadIdx.x;
+    u64 top = nx > ny ? nx : ny;
+    while (top > 0) {
+        const u64 lo = top > 256 ? top - 256 : 0;
+        __syncthreads();
+        if (t == 0) *sh = 0;
+        __syncthreads();
+        const u64 k = lo + t;
+        if (k < top) {
+            u64 a = k < nx ? x[k] : 0, b = k < ny ? y[k] : 0;
+            if (a != b) atomicMax(sh, (u32)(k - lo + 1));
+        }
+        __syncthreads();
+        const u32 h = *sh;
+        if (h) {
+            const u64 kk = lo + h - 1;
+            u64 a = kk < nx ? x[kk] : 0, b = kk < ny ? y[kk] : 0;
+            return a > b ? 1 : -1;
+        }
+        top = lo;
+    }
+    return 0;
+}
+
+// out[0..n_out) = a * b (+ add) by column sums (3 words each) and the CRT
+// carry lookahead in columns mode; n_out <= 4096 and the value must fit.
+__device__ void wg_mul(u64* out, u64 n_out, const u64* a, u64 na, const u64* b, u64 nb, const u64* add, u64 n_add,
+                       u64* c0, u64* c1, u64* c2, u64 (*xs)[256 + 3], u32 (*wco)[4], u8* fn) {
+    const u64 nc = na + nb - 1;
+    for (u64 col = threadIdx.x; col < nc; col += 256) {
+        const u64 i0 = col >= nb - 1 ? col - (nb - 1) : 0;
+        const u64 i1 = col < na - 1 ? col : na - 1;
+        u64 lo = 0, hi = 0, top = 0;
+        for (u64 i = i0; i <= i1; i++) {
+            u64 pl, ph;
+            mul_wide(a[i], b[col - i], pl, ph);
+            lo += pl;
+            u64 c = lo < pl ? 1 : 0;
+            hi += ph;
+            top += hi < ph ? 1 : 0;
+            hi += c;
+            top += hi < c ? 1 : 0;
+        }
+        c0[col] = lo;
+        c1[col] = hi;
+        c2[col] = top;
+    }
+    __syncthreads();
+    CrtArgs A;
+    A.r[0] = c0;
+    A.r[1] = c1;
+    A.r[2] = c2;
+    A.columns = 1;
+    A.n_res = nc;
+    A.out = out;
+    A.n_out = n_out;
+    A.addend = add;
+    A.n_add = n_add;
+    A.tile_fn = nullptr;
+    A.C = CrtConsts{};
+    crt_tile_body(A, 0, fn, xs, wco);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// e by binary splitting: num/den of a range [b, e) satisfy
+//   leaf [b, b+2): num = b+2, den = (b+1) b (den = 1 for b = 0),
+//   combine: num = num_l den_r + num_r, den = den_l den_r,
+// which is associative, so a block of pairs folds right to left with
+// one-word multipliers.
+// ---------------------------------------------------------------------------
+constexpr int E_LEAF_WORDS = 14;
+
+__global__ __launch_bounds__(256) void k_e_leaves(u64* num, u64* den, u64 nblocks, u64 stride) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nblocks) return;
+    const u64 b0 = 32 * j;
+    u64 N[E_LEAF_WORDS], D[E_LEAF_WORDS];
+#pragma unroll
+    for (int k = 0; k < E_LEAF_WORDS; k++) N[k] = D[k] = 0;
+    N[0] = b0 + 32;
+    D[0] = (b0 + 31) * (b0 + 30);
+#pragma unroll
+    for (int p = 14; p >= 0; p--) {
+        const u64 beg = b0 + 2 * (u64)p;
+        const u64 np = beg + 2;
+        const u64 dp = beg ? (beg + 1) * beg : 1;
+        u64 c1 = 0, c2 = 0;
+#pragma unroll
+        for (int k = 0; k < E_LEAF_WORDS; k++) {
+            u64 lo, hi;
+            mul_wide(np, D[k], lo, hi);
+            u64 s = lo + N[k];
+            hi += s < lo ? 1 : 0;
+            u64 s2 = s + c1;
+            hi += s2 < s ? 1 : 0;
+            N[k] = s2;
+            c1 = hi;
+            mul_wide(dp, D[k], lo, hi);
+            u64 d2 = lo + c2;
+            hi += d2 < lo ? 1 : 0;
+            D[k] = d2;
+            c2 = hi;
+        }
+    }
+    u64* on = num + j * stride;
+    u64* od = den + j * stride;
+#pragma unroll
+    for (int k = 0; k < E_LEAF_WORDS; k++)
+        if ((u64)k < stride) {
+            on[k] = N[k];
+            od[k] = D[k];
+        }
+    for (u64 k = E_LEAF_WORDS; k < stride; k++) on[k] = od[k] = 0;
+}
+
+constexpr int E_WG_MAX = 256;
+
+__global__ __launch_bounds__(256) void k_e_combine(const u64* __restrict__ num, const u64* __restrict__ den, u64 S,
+                                                   u64* num2, u64* den2, u64 S2, u64 ncomb) {
+    __shared__ u64 nl[E_WG_MAX], dl[E_WG_MAX], nr[E_WG_MAX], dr[E_WG_MAX];
+    __shared__ u64 c0[2 * E_WG_MAX], c1[2 * E_WG_MAX], c2[2 * E_WG_MAX];
+    __shared__ u64 xs[3][256 + 3];
+    __shared__ u32 wco[2][4];
+    __shared__ u8 fn;
+    const u32 t = threadIdx.x;
+    for (u64 c = blockIdx.x; c < ncomb; c += gridDim.x) {
+        const u64 L = 2 * c * S, R = (2 * c + 1) * S;
+        __syncthreads();
+        for (u64 k = t; k < S; k += 256) {
+            nl[k] = num[L + k];
+            dl[k] = den[L + k];
+            nr[k] = num[R + k];
+            dr[k] = den[R + k];
+        }
+        __syncthreads();
+        wg_mul(num2 + c * S2, S2, nl, S, dr, S, nr, S, c0, c1, c2, xs, wco, &fn);
+        wg_mul(den2 + c * S2, S2, dl, S, dr, S, nullptr, 0, c0, c1, c2, xs, wco, &fn);
+    }
+}
+
+void launch_e_leaves(u64* num, u64* den, u64 nblocks, u64 stride, hipStream_t st) {
+    hipLaunchKernelGGL(k_e_leaves, dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, st, num, den, nblocks,
+                       stride);
+}
+void launch_e_combine(const u64* num, const u64* den, u64 S, u64* num2, u64* den2, u64 S2, u64 ncomb,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_e_combine, dim3(persist_grid(ncomb, 16384)), dim3(256), 0, st, num, den, S, num2, den2, S2,
+                       ncomb);
+}
+
+// ---------------------------------------------------------------------------
+// Radix conversion.
+// Division of a two-word numerator by 10^18 with a precomputed reciprocal:
+// Moller and Granlund, "Improved division by invariant integers" (IEEE
+// Trans. Computers 60(2), 2011), Algorithm 4, on the normalised divisor
+// d = 10^18 * 2^4 with v = floor((2^128 - 1) / d) - 2^64.
+// ---------------------------------------------------------------------------
+constexpr u64 DEC_DN = 0xde0b6b3a76400000ull;   // 10^18 << 4
+constexpr u64 DEC_V = 0x2725dd1d243aba0eull;
+
+// (u1 * 2^64 + u0) / d for normalised d (top bit set), u1 < d, and
+// v = floor((2^128 - 1) / d) - 2^64: returns the quotient, r <- remainder.
+__device__ __forceinline__ u64 div_preinv(u64 u1, u64 u0, u64 d, u64 v, u64& r) {
+    u64 q0, q1;
+    mul_wide(v, u1, q0, q1);
+    u64 s = q0 + u0;
+    q1 += (s < q0 ? 1 : 0) + u1 + 1;
+    q0 = s;
+    u64 rr = u0 - q1 * d;
+    if (rr > q0) {
+        q1--;
+        rr += d;
+    }
+    if (rr >= d) {
+        q1++;
+        rr -= d;
+    }
+    r = rr;
+    return q1;
+}
+
+// (r * 2^64 + u) / 10^18 for r < 10^18: returns the quotient word, r <- remainder.
+__device__ __forceinline__ u64 div_1e18(u64& r, u64 u) {
+    u64 rr;
+    u64 q = div_preinv((r << 4) | (u >> 60), u << 4, DEC_DN, DEC_V, rr);
+    r = rr >> 4;
+    return q;
+}
+
+// q[0..n) = a / d0, *rem = a mod d0 for a one-word divisor (reference
+// div_base, natural.h:566-574); d = d0 << l normalised.  A single thread
+// walks the words from the top (the remainder chain is sequential).
+__global__ void k_div_word(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64* rem) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    u64 r = l ? a[n - 1] >> (64 - l) : 0;
+    for (u64 k = n; k-- > 0;) {
+        u64 u0 = a[k] << l;
+        if (l && k > 0) u0 |= a[k - 1] >> (64 - l);
+        q[k] = div_preinv(r, u0, d, v, r);
+    }
+    *rem = r >> l;
+}
+
+void launch_div_word(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64* rem, hipStream_t st) {
+    hipLaunchKernelGGL(k_div_word, dim3(1), dim3(64), 0, st, q, a, n, d, l, v, rem);
+}
+
+__device__ __forceinline__ void put9(char* o, u32 v) {
+#pragma unroll
+    for (int d = 8; d >= 0; d--) {
+        o[d] = (char)('0' + v % 10u);
+        v /= 10u;
+    }
+}
+__device__ __forceinline__ void put18(char* o, u64 v) {
+    u64 hi = v / 1000000000ull;
+    put9(o, (u32)hi);
+    put9(o + 9, (u32)(v - hi * 1000000000ull));
+}
+
+constexpr int DEC_LEAF_WORDS = 16;
+
+__global__ __launch_bounds__(256) void k_dec_leaves(const u64* __restrict__ x, u64 stride, u64 count, int nchunks,
+                                                    char* out) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    u64 v[DEC_LEAF_WORDS];
+#pragma unroll
+    for (int k = 0; k < DEC_LEAF_WORDS; k++) v[k] = (u64)k < stride ? x[j * stride + k] : 0;
+    char* o = out + j * 18 * (u64)nchunks;
+#pragma unroll
+    for (int c = 0; c < DEC_LEAF_WORDS; c++) {
+        if (c < nchunks) {
+            u64 r = 0;
+#pragma unroll
+            for (int k = DEC_LEAF_WORDS - 1; k >= 0; k--) v[k] = div_1e18(r, v[k]);
+            put18(o + 18 * (nchunks - 1 - c), r);
+        }
+    }
+}
+
+constexpr int DEC_WG_MAX = 256;  // largest base (words) split by one work-group
+
+__global__ __launch_bounds__(256) void k_dec_split_wg(const u64* __restrict__ x, u64 sx, u64 count,
+                                                      const u64* __restrict__ base, u64 w, const u64* __restrict__ r,
+                                                      u64 rn, u32 n_shl, u64* out, u64* err) {
+    __shared__ u64 X[2 * DEC_WG_MAX + 16];
+    __shared__ u64 Rr[DEC_WG_MAX + 16];
+    __shared__ u64 Bs[DEC_WG_MAX];
+    __shared__ u64 Pq[2 * DEC_WG_MAX + 32];
+    __shared__ u64 H[DEC_WG_MAX];
+    __shared__ u64 c0[2 * DEC_WG_MAX + 32], c1[2 * DEC_WG_MAX + 32], c2[2 * DEC_WG_MAX + 32];
+    __shared__ u64 xs[3][256 + 3];
+    __shared__ u32 wco[2][4];
+    __shared__ u32 sh;
+    __shared__ u8 fn;
+    const u32 t = threadIdx.x;
+    for (u64 k = t; k < rn; k += 256) Rr[k] = r[k];
+    for (u64 k = t; k < w; k += 256) Bs[k] = base[k];
+    const u64 xn = sx - (w - 1);               // words of x >> (w-1) words
+    const u64 sbits = (w + 2) * 64 - n_shl;    // hi = (xt * r) >> sbits
+    const u64 ws = sbits >> 6;
+    const u32 bs = (u32)(sbits & 63);
+    const u64 pn = xn + rn;
+    for (u64 j = blockIdx.x; j < count; j += gridDim.x) {
+        __syncthreads();
+        for (u64 k = t; k < 2 * w; k += 256) X[k] = k < sx ? x[j * sx + k] : 0;
+        __syncthreads();
+        wg_mul(Pq, pn, X + (w - 1), xn, Rr, rn, nullptr, 0, c0, c1, c2, xs, wco, &fn);
+        for (u64 k = t; k < w; k += 256) {
+            u64 i = k + ws;
+            u64 lo = i < pn ? Pq[i] >> bs : 0;
+            u64 hi = (bs && i + 1 < pn) ? Pq[i + 1] << (64 - bs) : 0;
+            H[k] = lo | hi;
+        }
+        __syncthreads();
+        wg_mul(Pq, 2 * w, H, w, Bs, w, nullptr, 0, c0, c1, c2, xs, wco, &fn);
+        u32 nb = wg_addsub(X, 2 * w, Pq, 2 * w, 1, 1, wco);  // lo = x - hi*base
+        int bad = nb ? 0 : 1;
+        for (int it = 0; it < 8 && wg_cmp(X, 2 * w, Bs, w, &sh) >= 0; it++) {
+            wg_addsub(X, 2 * w, Bs, w, 1, 1, wco);
+            wg_addsub(H, w, nullptr, 0, 0, 1, wco);
+            if (it == 7) bad = 1;
+        }
+        if (bad && t == 0) *err = 1;
+        for (u64 k = t; k < w; k += 256) {
+            out[(2 * j) * w + k] = H[k];
+            out[(2 * j + 1) * w + k] = X[k];
+        }
+    }
+}
+
+__global__ void k_shr_pieces(u64* out, u64 so, const u64* in, u64 si, u64 n_in, u64 shift, u64 n_out, u64 count) {
+    const u64 ws = shift >> 6;
+    const u32 bs = (u32)(shift & 63);
+    const u64 total = n_out * count;
+    for (u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (u64)gridDim.x * blockDim.x) {
+        const u64 j = g / n_out, k = g - j * n_out;
+        const u64* p = in + j * si;
+        const u64 i = k + ws;
+        u64 lo = i < n_in ? p[i] >> bs : 0;
+        u64 hi = (bs && i + 1 < n_in) ? p[i + 1] << (64 - bs) : 0;
+        out[j * so + k] = lo | hi;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cmp_pieces(const u64* x, u64 sx, u64 nx, const u64* y, u64 ny, u64 count,
+                                                    u8* flags) {
+    __shared__ u32 sh;
+    for (u64 j = blockIdx.x; j < count; j += gridDim.x) {
+        int c = wg_cmp(x + j * sx, nx, y, ny, &sh);
+        if (threadIdx.x == 0) flags[j] = c >= 0 ? 1 : 0;
+    }
+}
+
+void launch_dec_leaves(const u64* x, u64 stride, u64 count, int lvl, char* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_dec_leaves, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, x, stride, count,
+                       1 << (lvl + 1), out);
+}
+void launch_dec_split_wg(const u64* x, u64 sx, u64 count, const u64* base, u64 w, const u64* r, u64 rn, u32 n_shl,
+                         u64* out, u64* err, hipStream_t st) {
+    hipLaunchKernelGGL(k_dec_split_wg, dim3(persist_grid(count, 16384)), dim3(256), 0, st, x, sx, count, base, w, r,
+                       rn, n_shl, out, err);
+}
+void launch_shr_pieces(u64* out, u64 so, const u64* in, u64 si, u64 n_in, u64 shift, u64 n_out, u64 count,
+                       hipStream_t st) {
+    u64 total = n_out * count;
+    if (total)
+        hipLaunchKernelGGL(k_shr_pieces, dim3(grid_for(total)), dim3(256), 0, st, out, so, in, si, n_in, shift, n_out,
+                           count);
+}
+void launch_cmp_pieces(const u64* x, u64 sx, u64 nx, const u64* y, u64 ny, u64 count, u8* flags, hipStream_t st) {
+    hipLaunchKernelGGL(k_cmp_pieces, dim3(persist_grid(count, 16384)), dim3(256), 0, st, x, sx, nx, y, ny, count,
+                       flags);
+}
+
+}  // namespace nnh

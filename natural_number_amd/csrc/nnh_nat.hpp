@@ -1,0 +1,72 @@
+// nnh_nat.hpp — device-side limb arithmetic used by the device-resident
+// natural layer (division, radix conversion, binary splitting).
+#pragma once
+#include "nnh_internal.hpp"
+
+namespace nnh {
+
+// One modular addition of a batch: out[0..n) = a + b (+ carry-in) or
+// a - b (a + ~b + 1), with a zero-extended beyond na and b beyond nb; the
+// carry out of word n-1 is dropped (stored to *cout when non-null: for a
+// subtraction 1 means no borrow).  out may alias a or b.
+struct AddJob {
+    u64* out;
+    const u64* a;
+    const u64* b;
+    u64 n, na, nb;
+    u64 tile0;            // first 4096-word tile of this job (filled by the launcher)
+    const u32* cin_ptr;   // device carry-in (add only), overrides cin when non-null
+    u32* cout;
+    u32 sub;
+    u32 cin;
+};
+
+constexpr u64 LIMB_TILE = 4096;
+
+// Enqueue a batch of additions; jobs/tile scratch are device buffers of at
+// least njobs AddJobs and 2*ntiles bytes (ntiles = sum ceil(n/4096)).
+u64 add_tiles_needed(const AddJob* jobs, u64 njobs);
+void launch_add_batch(AddJob* d_jobs, u32 njobs, u64 ntiles, u8* d_tiles, hipStream_t st);
+
+void launch_not(u64* out, const u64* a, u64 n, hipStream_t st);
+void launch_fill(u64* out, u64 n, u64 v, hipStream_t st);
+// out[0..n] = a << b (1 <= b <= 63): n + 1 words.
+void launch_shl_bits(u64* out, const u64* a, u64 n, u32 b, hipStream_t st);
+// out[0..n) = a >> b (1 <= b <= 63).
+void launch_shr_bits(u64* out, const u64* a, u64 n, u32 b, hipStream_t st);
+// Highest index k < n with: mode 0 a[k] != 0, mode 1 a[k] != ~0,
+// mode 2 a[k] != b[k].  res3 (device, 3 words) receives {k+1 or 0,
+// a[k], b[k]} (zeros when no index qualifies).
+void launch_find(const u64* a, const u64* b, u64 n, int mode, u64* res3, hipStream_t st);
+void launch_set_word(u64* p, u64 v, hipStream_t st);
+// q[0..n) = a / d0 and *rem = a mod d0, where d = d0 << l has its top bit
+// set and v = floor((2^128 - 1) / d) - 2^64.
+void launch_div_word(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64* rem, hipStream_t st);
+
+// Binary splitting of sum 1/k! (reference calc_e, natural.h:841-857).
+// Leaves: thread j folds the 16 pairs [32j, 32j+32) into (num, den) at
+// stride words.  Requires stride >= the value's words (<= 14 for terms < 2^25).
+void launch_e_leaves(u64* num, u64* den, u64 nblocks, u64 stride, hipStream_t st);
+// One level: combine c folds ranges 2c and 2c+1 (stride S <= 256) into
+// num2 = num_l den_r + num_r, den2 = den_l den_r (stride S2 <= 4096).
+void launch_e_combine(const u64* num, const u64* den, u64 S, u64* num2, u64* den2, u64 S2, u64 ncomb,
+                      hipStream_t st);
+
+// Radix conversion (reference _dec_split/_print_dec, natural.h:953-992).
+// Leaves: piece j (< 10^(18*2^(lvl+1)), stride words <= 15) becomes
+// 18*2^(lvl+1) zero-padded digits at out + j * that many chars.
+void launch_dec_leaves(const u64* x, u64 stride, u64 count, int lvl, char* out, hipStream_t st);
+// One level of splits by base = 10^(18*2^i) with w = |base| <= 256 words:
+// piece j (stride sx words) -> hi at out[(2j) w], lo at out[(2j+1) w],
+// using the reference's reciprocal r (rn words) and normalising shift n_shl.
+// err (device word) is set nonzero when an estimate exceeds the quotient.
+void launch_dec_split_wg(const u64* x, u64 sx, u64 count, const u64* base, u64 w, const u64* r, u64 rn, u32 n_shl,
+                         u64* out, u64* err, hipStream_t st);
+// Batched helpers for the large levels of the conversion.
+// out_j[0..n_out) = (in_j >> shift bits), pieces at strides so / si with n_in words.
+void launch_shr_pieces(u64* out, u64 so, const u64* in, u64 si, u64 n_in, u64 shift, u64 n_out, u64 count,
+                       hipStream_t st);
+// flags[j] = (x_j >= y) for pieces x_j (stride sx, nx words) and one y (ny words).
+void launch_cmp_pieces(const u64* x, u64 sx, u64 nx, const u64* y, u64 ny, u64 count, u8* flags, hipStream_t st);
+
+}  // namespace nnh

@@ -1,0 +1,827 @@
+// nnh_natural.cpp — device-resident naturals and the reference algorithms built
+// on the GPU multiply: Newton division (natural.h:566-769), pow
+// (natural.h:554-564), decimal output (natural.h:904-1039) and the e
+// workload (calc_e natural.h:841-857, recipe test.cpp:12-19).
+//
+// Limbs live in device memory from input to output.  The host walks the
+// reference's control flow, which depends on operand sizes and a few
+// leading words; those are read back through small pinned copies.  Every
+// product, addition, shift and comparison runs as a device kernel.  Sizes
+// follow the reference exactly (normalised where it normalises, structural
+// where it resizes), because they steer the Newton iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <bit>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/nnh.h"
+#include "nnh_host.hpp"
+#include "nnh_nat.hpp"
+
+namespace nnh {
+namespace {
+
+typedef unsigned __int128 u128;
+
+constexpr u64 NTT_DIV_THRESHOLD = 256;   // ntt_info.h:7
+constexpr int NTT_SCALE_THRESHOLD = 10;  // ntt_info.h:8
+constexpr u64 DEC_BASE0 = 1000000000000000000ull;  // natural.h:904
+constexpr int DEC_WG_MAX_WORDS = 256;    // bases split by one work-group (kernel limit)
+constexpr int DEC_LEAF_LEVEL = 3;        // pieces < 10^(18*16) become digits per thread
+constexpr int E_WG_MAX_WORDS = 256;      // e levels combined by one work-group
+constexpr int E_LEAF_MAX_WORDS = 14;
+
+int ntt_scale(u64 s1, u64 s2) { return 64 - std::countl_zero(s1 + s2 - 2); }  // natural.h:384-386
+
+struct Err {
+    int code;
+    std::string msg;
+};
+[[noreturn]] void raise(int code, const std::string& m) { throw Err{code, m}; }
+void hipchk(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        raise(e == hipErrorOutOfMemory ? NNH_ERR_NOMEM : NNH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+void rcchk(int rc) {
+    if (rc) throw Err{rc, nnh_last_error()};
+}
+
+// Device buffers recycled by power-of-two capacity; all released when the
+// pool dies.  Reuse is safe because every kernel runs on one stream.
+class Pool {
+   public:
+    ~Pool() {
+        (void)hipDeviceSynchronize();
+        for (u64* p : all_) (void)hipFree(p);
+    }
+    u64* get(u64 words, u64& cap) {
+        cap = std::bit_ceil(std::max<u64>(words, 64));
+        auto it = free_.find(cap);
+        if (it != free_.end()) {
+            u64* p = it->second;
+            free_.erase(it);
+            return p;
+        }
+        u64* p = nullptr;
+        hipchk(hipMalloc((void**)&p, cap * sizeof(u64)), "device allocation");
+        all_.push_back(p);
+        return p;
+    }
+    void put(u64* p, u64 cap) { free_.emplace(cap, p); }
+
+   private:
+    std::multimap<u64, u64*> free_;
+    std::vector<u64*> all_;
+};
+
+struct Block {
+    Pool* pool;
+    u64* p;
+    u64 cap;
+    Block(Pool* pl, u64* pp, u64 c) : pool(pl), p(pp), cap(c) {}
+    Block(const Block&) = delete;
+    Block& operator=(const Block&) = delete;
+    ~Block() { pool->put(p, cap); }
+};
+
+// A natural in device memory: words [p, p + n), n >= 1.  Views share the
+// owning block.
+struct DN {
+    std::shared_ptr<Block> blk;
+    u64* p = nullptr;
+    u64 n = 0;
+    DN view(u64 off, u64 len) const {
+        DN v = *this;
+        v.p = p + off;
+        v.n = len;
+        return v;
+    }
+};
+
+// Transform kept for reuse (NTT_info in mul_save/mul_load_NTT_info):
+// its scale and the size of the operand it holds.
+struct Xf {
+    int scale = 0;
+    u64 held_n = 0;
+};
+
+class Ctx {
+   public:
+    explicit Ctx(hipStream_t s) : st(s) {
+        hipchk(hipHostMalloc((void**)&h_, 4096, hipHostMallocDefault), "pinned allocation");
+        res_ = alloc(8);
+    }
+    ~Ctx() {
+        (void)hipStreamSynchronize(st);
+        if (h_) (void)hipHostFree(h_);
+        if (d_jobs_) (void)hipFree(d_jobs_);
+        if (d_tiles_) (void)hipFree(d_tiles_);
+        if (h_jobs_) (void)hipHostFree(h_jobs_);
+    }
+    hipStream_t st;
+    Pool pool;
+
+    DN alloc(u64 n) {
+        u64 cap;
+        u64* p = pool.get(n, cap);
+        DN x;
+        x.blk = std::make_shared<Block>(&pool, p, cap);
+        x.p = p;
+        x.n = n;
+        return x;
+    }
+    DN zeros(u64 n) {
+        DN x = alloc(n);
+        launch_fill(x.p, n, 0, st);
+        return x;
+    }
+    DN word(u64 v) {
+        DN x = alloc(1);
+        launch_set_word(x.p, v, st);
+        return x;
+    }
+    DN upload(const u64* h, u64 n) {
+        DN x = alloc(n);
+        hipchk(hipMemcpyAsync(x.p, h, n * sizeof(u64), hipMemcpyHostToDevice, st), "upload");
+        sync();
+        return x;
+    }
+    void download(u64* h, const u64* d, u64 n) {
+        hipchk(hipMemcpyAsync(h, d, n * sizeof(u64), hipMemcpyDeviceToHost, st), "download");
+        sync();
+    }
+    void copy(u64* dst, const u64* src, u64 n) {
+        if (n) hipchk(hipMemcpyAsync(dst, src, n * sizeof(u64), hipMemcpyDeviceToDevice, st), "copy");
+    }
+    void sync() {
+        hipchk(hipGetLastError(), "launch");
+        hipchk(hipStreamSynchronize(st), "synchronise");
+    }
+    u64 read(const u64* p) {
+        hipchk(hipMemcpyAsync(h_, p, sizeof(u64), hipMemcpyDeviceToHost, st), "read");
+        sync();
+        return h_[0];
+    }
+    u64 top(const DN& x) { return read(x.p + x.n - 1); }
+    // 1 + highest qualifying index (0: none); wa/wb receive the words there.
+    u64 find(const u64* a, const u64* b, u64 n, int mode, u64* wa = nullptr, u64* wb = nullptr) {
+        launch_find(a, b, n, mode, res_.p, st);
+        hipchk(hipMemcpyAsync(h_, res_.p, 3 * sizeof(u64), hipMemcpyDeviceToHost, st), "find");
+        sync();
+        if (wa) *wa = h_[1];
+        if (wb) *wb = h_[2];
+        return h_[0];
+    }
+    // natural.h:82-93
+    void normalize(DN& x) {
+        if (x.n == 0) {
+            x = word(0);
+            return;
+        }
+        u64 h = find(x.p, nullptr, x.n, 0);
+        x.n = h ? h : 1;
+    }
+    bool is_zero(const DN& x) { return x.n == 1 && read(x.p) == 0; }
+    // Compare normalised naturals (natural.h:145-160).
+    int cmp(const DN& a, const DN& b) {
+        if (a.n != b.n) return a.n < b.n ? -1 : 1;
+        u64 wa = 0, wb = 0;
+        u64 h = find(a.p, b.p, a.n, 2, &wa, &wb);
+        if (!h) return 0;
+        return wa > wb ? 1 : -1;
+    }
+
+    // ---- additions (base.h:7-16, asmAdd1) ----------------------------------
+    void run_adds(std::vector<AddJob>& jobs) {
+        if (jobs.empty()) return;
+        u64 tiles = 0;
+        for (AddJob& J : jobs) {
+            J.tile0 = tiles;
+            tiles += (J.n + LIMB_TILE - 1) / LIMB_TILE;
+        }
+        u64 jb = jobs.size() * sizeof(AddJob);
+        if (jb > jobs_cap_) {
+            if (d_jobs_) (void)hipFree(d_jobs_);
+            if (h_jobs_) (void)hipHostFree(h_jobs_);
+            d_jobs_ = nullptr;
+            h_jobs_ = nullptr;
+            jobs_cap_ = std::bit_ceil(std::max<u64>(jb, 4096));
+            hipchk(hipMalloc((void**)&d_jobs_, jobs_cap_), "job table");
+            hipchk(hipHostMalloc((void**)&h_jobs_, jobs_cap_, hipHostMallocDefault), "job staging");
+        }
+        if (2 * tiles > tiles_cap_) {
+            if (d_tiles_) (void)hipFree(d_tiles_);
+            tiles_cap_ = std::bit_ceil(std::max<u64>(2 * tiles, 4096));
+            hipchk(hipMalloc((void**)&d_tiles_, tiles_cap_), "tile flags");
+        }
+        std::memcpy(h_jobs_, jobs.data(), jb);
+        hipchk(hipMemcpyAsync(d_jobs_, h_jobs_, jb, hipMemcpyHostToDevice, st), "job upload");
+        launch_add_batch(d_jobs_, (u32)jobs.size(), tiles, d_tiles_, st);
+        sync();  // the staging buffer is reused by the next call
+    }
+    // dst[0..as) = a + b (as >= bs), the carry dropped or stored to *cout.
+    void add_to(u64* dst, const u64* a, u64 as, const u64* b, u64 bs, u32* cout = nullptr,
+                const u32* cin_ptr = nullptr) {
+        if (bs > as) raise(NNH_ERR_DOMAIN, "addition with a longer second operand");
+        std::vector<AddJob> j(1);
+        j[0] = AddJob{dst, a, b, as, as, bs, 0, cin_ptr, cout, 0, 0};
+        run_adds(j);
+    }
+    // dst[0..as) = a - b (as >= bs), the borrow dropped.
+    void sub_from(u64* dst, const u64* a, u64 as, const u64* b, u64 bs) {
+        if (bs > as) raise(NNH_ERR_DOMAIN, "subtraction with a longer second operand");
+        std::vector<AddJob> j(1);
+        j[0] = AddJob{dst, a, b, as, as, bs, 0, nullptr, nullptr, 1, 0};
+        run_adds(j);
+    }
+    // natural.h:303-311
+    DN add(const DN& x, const DN& y) {
+        const DN& a = x.n >= y.n ? x : y;
+        const DN& b = x.n >= y.n ? y : x;
+        DN d = alloc(a.n + 1);
+        std::vector<AddJob> j(1);
+        j[0] = AddJob{d.p, a.p, b.p, a.n + 1, a.n, b.n, 0, nullptr, nullptr, 0, 0};
+        run_adds(j);
+        normalize(d);
+        return d;
+    }
+    // natural.h:324-342 for x >= y
+    DN sub(const DN& x, const DN& y) {
+        DN d = alloc(x.n);
+        std::vector<AddJob> j(1);
+        j[0] = AddJob{d.p, x.p, y.p, x.n, x.n, y.n, 0, nullptr, nullptr, 1, 0};
+        run_adds(j);
+        normalize(d);
+        return d;
+    }
+
+    // ---- shifts (natural.h:264-295) ------------------------------------------
+    // Word shifts keep the size structural; bit shifts normalise.  a != 0.
+    DN shl(const DN& a, u64 bits) {
+        u64 n = bits >> 6;
+        u32 b = (u32)(bits & 63);
+        if (b == 0) {
+            DN d = alloc(a.n + n);
+            launch_fill(d.p, n, 0, st);
+            copy(d.p + n, a.p, a.n);
+            return d;
+        }
+        DN d = alloc(a.n + n + 1);
+        launch_fill(d.p, n, 0, st);
+        launch_shl_bits(d.p + n, a.p, a.n, b, st);
+        normalize(d);
+        return d;
+    }
+    DN shr(const DN& a, u64 bits) {
+        u64 n = bits >> 6;
+        if (a.n <= n) return word(0);
+        u32 b = (u32)(bits & 63);
+        if (b == 0) return a.view(n, a.n - n);
+        DN d = alloc(a.n - n);
+        launch_shr_bits(d.p, a.p + n, a.n - n, b, st);
+        normalize(d);
+        return d;
+    }
+    DN complement(const u64* p, u64 n) {
+        DN d = alloc(n);
+        launch_not(d.p, p, n, st);
+        return d;
+    }
+
+    // ---- products ------------------------------------------------------------
+    DN mul(const DN& a, const DN& b) {
+        DN d = alloc(a.n + b.n);
+        rcchk(dev_mul(d.p, a.p, a.n, b.p, b.n, st));
+        normalize(d);
+        return d;
+    }
+    DN sqr(const DN& a) {
+        DN d = alloc(2 * a.n);
+        rcchk(dev_sqr(d.p, a.p, a.n, st));
+        normalize(d);
+        return d;
+    }
+    // NTT_info load/mul/INTT/save at scale s (ntt_info.h:147-248).
+    DN mul_scale(const DN& a, const DN& b, int s, u64 save_size) {
+        u64 L = 1ull << s;
+        DN d = alloc(std::max(a.n + b.n, L + 2));
+        u64 n = 0;
+        rcchk(dev_mul_scale(d.p, a.p, a.n, b.p, b.n, s, save_size, &n, st));
+        d.n = n;
+        normalize(d);
+        return d;
+    }
+    // mul_save_NTT_info (natural.h:452-467): d = a * b, x keeps a's transform.
+    DN mul_save(Xf& x, const DN& a, const DN& b) {
+        if (x.scale < NTT_SCALE_THRESHOLD) {
+            x.scale = 0;
+            return mul(a, b);
+        }
+        x.held_n = a.n;
+        return mul_scale(a, b, x.scale, a.n + b.n);
+    }
+    // mul_load_NTT_info (natural.h:504-515): d = a * b with a's kept transform.
+    DN mul_load(const Xf& x, const DN& a, const DN& b) {
+        if (x.scale == 0) return mul(a, b);
+        if (a.n != x.held_n) raise(NNH_ERR_DOMAIN, "kept transform holds a different operand");
+        return mul_scale(a, b, x.scale, a.n + b.n);
+    }
+
+   private:
+    u64* h_ = nullptr;
+    DN res_;
+    AddJob* d_jobs_ = nullptr;
+    AddJob* h_jobs_ = nullptr;
+    u64 jobs_cap_ = 0;
+    u8* d_tiles_ = nullptr;
+    u64 tiles_cap_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Division (natural.h:566-769)
+// ---------------------------------------------------------------------------
+// Start estimate of both reciprocals (natural.h:579-583, 622-624).
+DN initial_estimate(Ctx& c, const DN& d) {
+    u64 w = c.read(d.p + d.n - 2);
+    u64 v = (1ull << 63) | (w >> 1);
+    if (v == ~0ull) {
+        u64 h[2] = {1ull << 63, 1};
+        return c.upload(h, 2);
+    }
+    return c.word((u64)((((u128)1 << 63) << 64) / ((u128)v + 1)));
+}
+
+// natural.h:576-617 (reciprocal without a cached divisor transform)
+DN reciprocal_plain(Ctx& c, const DN& divs, u64 divd_size) {
+    u64 expected = divd_size + 1 - divs.n;
+    DN ans = initial_estimate(c, divs);
+    for (;;) {
+        u64 cur = ans.n;
+        if (cur >= expected + 1) return c.shr(ans, (cur - expected) << 6);
+        DN tmp = divs.n > 2 * cur + 1 ? c.shr(divs, (divs.n - 2 * cur - 1) << 6)
+                                      : c.shl(divs, (2 * cur + 1 - divs.n) << 6);
+        Xf xa;
+        xa.scale = ntt_scale(tmp.n, 1);
+        tmp = c.mul_save(xa, ans, tmp);
+        if (tmp.n <= cur) raise(NNH_ERR_DOMAIN, "reciprocal: product shorter than the estimate");
+        tmp = c.complement(tmp.p + cur, tmp.n - cur);
+        c.normalize(tmp);
+        int lz = std::countl_zero(c.top(tmp));
+        u64 nxt = std::min(2 * cur, (2 * cur - tmp.n) * 2 + (u64)((lz + 31) >> 5));
+        if (nxt == NTT_DIV_THRESHOLD) nxt = NTT_DIV_THRESHOLD - 1;
+        if (nxt <= cur)  // no progress: the reference would shift by a negative count
+            raise(NNH_ERR_DOMAIN, "reciprocal: accuracy did not grow");
+        u64 drop = 2 * cur - nxt;
+        tmp = tmp.n <= drop ? c.word(1) : c.shr(tmp, drop << 6);
+        tmp = c.mul_load(xa, ans, tmp);
+        ans = c.shl(ans, (nxt - cur) << 6);
+        if (tmp.n > cur) c.add_to(ans.p, ans.p, ans.n, tmp.p + cur, tmp.n - cur);
+    }
+}
+
+// natural.h:620-687 (reciprocal keeping the divisor transform in xdiv)
+DN reciprocal_ntt(Ctx& c, const DN& divisor, u64 dividend_size, Xf& xdiv) {
+    u64 expected = dividend_size + 1 - divisor.n;
+    DN ans = initial_estimate(c, divisor);
+    for (u64 i = 1; i < divisor.n; i *= 2) {
+        u64 prev = ans.n, seg_size = 2 * prev + 1;
+        if (2 * i >= divisor.n) seg_size = divisor.n;
+        DN seg = divisor.view(divisor.n - seg_size, seg_size);
+        DN tmp;
+        Xf xans;
+        if (2 * i <= NTT_DIV_THRESHOLD) {
+            tmp = c.mul(seg, ans);
+        } else {
+            int s = ntt_scale(i, i);
+            xans.scale = s;
+            xans.held_n = ans.n;
+            xdiv.scale = s;
+            xdiv.held_n = seg_size;
+            tmp = c.mul_scale(ans, seg, s, seg_size + prev);
+        }
+        u64 n_shl = seg_size - 1 - prev;
+        if (tmp.n > n_shl + 1) {
+            u64 h = c.find(tmp.p + n_shl + 1, nullptr, tmp.n - n_shl - 1, 1);
+            tmp.n = n_shl + 1 + h;
+        }
+        int lz = std::countl_zero(~c.top(tmp));
+        u64 cur = std::min(2 * prev, (2 * prev - tmp.n + n_shl) * 2 + (u64)((lz + 31) >> 5));
+        if (cur == NTT_DIV_THRESHOLD) cur -= 1;
+        if (cur < prev)  // the reference's natural.h:675-677 then writes ~2^64 words
+            raise(NNH_ERR_DOMAIN, "divisor outside the reference reciprocal's domain");
+        n_shl += 2 * prev - cur;
+        if (tmp.n <= n_shl) {
+            tmp = c.word(1);
+        } else {
+            tmp = c.complement(tmp.p + n_shl, tmp.n - n_shl);
+            c.normalize(tmp);
+        }
+        tmp = c.mul_load(xans, ans, tmp);
+        tmp = c.shr(tmp, prev << 6);
+        u64 inc = cur - prev;
+        DN na = c.alloc(cur);
+        if (inc >= tmp.n) {
+            c.copy(na.p, tmp.p, tmp.n);
+            launch_fill(na.p + tmp.n, inc - tmp.n, 0, c.st);
+            c.copy(na.p + inc, ans.p, prev);
+        } else {
+            c.copy(na.p, tmp.p, inc);
+            c.add_to(na.p + inc, ans.p, prev, tmp.p + inc, tmp.n - inc);
+        }
+        ans = na;
+        if (cur >= expected + 1) return c.shr(ans, (cur - expected) << 6);
+    }
+    return c.shr(ans, 64);
+}
+
+// natural.h:689-747
+void div_iterative(Ctx& c, const DN& divd, const DN& divs, DN& q, DN& r) {
+    int n_shl = (std::countl_zero(c.top(divs)) + 1) & 63;
+    DN b = c.shl(divs, (u64)n_shl);
+    r = c.shl(divd, (u64)n_shl);
+    Xf xb;
+    DN rec = reciprocal_ntt(c, b, r.n, xb);
+    Xf xrec;
+    xrec.scale = ntt_scale(rec.n, rec.n);
+    int sb = ntt_scale(b.n, 1);
+    u64 Lb = 1ull << sb;
+    if (xb.scale != sb && sb >= NTT_SCALE_THRESHOLD - 1) {
+        xb.scale = sb;
+        xb.held_n = b.n;
+    }
+    q = c.zeros(r.n - b.n + 1);
+    bool first = true;
+    DN cf_word = c.zeros(1);  // carry between the two halves of the fold
+    u32* cf = (u32*)cf_word.p;
+    for (int iter = 0;; iter++) {
+        if (iter > 4096) raise(NNH_ERR_DOMAIN, "division: remainder did not shrink");
+        u64 idx = 0;
+        if (r.n > rec.n + b.n - 1) idx = r.n - rec.n - b.n + 1;
+        DN top = c.shr(r, (idx + b.n - 1) << 6);
+        DN tmp = first ? c.mul_save(xrec, rec, top) : c.mul_load(xrec, rec, top);
+        first = false;
+        tmp = c.shr(tmp, rec.n << 6);
+        c.add_to(q.p + idx, q.p + idx, q.n - idx, tmp.p, tmp.n);
+        tmp = c.mul_load(xb, b, tmp);
+        c.sub_from(r.p + idx, r.p + idx, r.n - idx, tmp.p, tmp.n);
+        c.normalize(r);
+        if (xb.scale) {
+            if (r.n - idx > Lb) {
+                c.add_to(r.p + idx, r.p + idx, Lb, r.p + idx + Lb, r.n - idx - Lb, cf);
+                c.add_to(r.p + idx, r.p + idx, Lb, nullptr, 0, nullptr, cf);
+                r.n = idx + Lb;
+            }
+            if (r.n == idx + Lb && (c.top(r) >> 63)) r.n = idx;
+            c.normalize(r);
+        }
+        if (idx == 0) break;
+    }
+    DN one = c.word(1);
+    for (int fix = 0; c.cmp(r, b) >= 0; fix++) {
+        if (fix >= 64) raise(NNH_ERR_DOMAIN, "division: final correction did not converge");
+        q = c.add(q, one);
+        r = c.sub(r, b);
+    }
+    r = c.shr(r, (u64)n_shl);
+}
+
+// natural.h:749-769 (_div): q and r normalised.
+void divmod(Ctx& c, const DN& a, const DN& b, DN& q, DN& r) {
+    if (c.is_zero(b)) {
+        q = c.word(c.is_zero(a) ? 1 : 0);
+        r = c.word(0);
+        return;
+    }
+    if (c.cmp(a, b) < 0) {
+        q = c.word(0);
+        r = a;
+        return;
+    }
+    if (b.n == 1) {
+        u64 d0 = c.read(b.p);
+        u32 l = (u32)std::countl_zero(d0);
+        u64 d = d0 << l;
+        u64 v = (u64)((~(u128)0) / d - ((u128)1 << 64));
+        q = c.alloc(a.n);
+        DN rem = c.alloc(1);
+        launch_div_word(q.p, a.p, a.n, d, l, v, rem.p, c.st);
+        c.normalize(q);
+        r = rem;
+        return;
+    }
+    div_iterative(c, a, b, q, r);
+    c.normalize(q);
+}
+
+// natural.h:554-564
+DN pow(Ctx& c, const DN& b, u64 e) {
+    if (e == 0) return c.word(1);
+    if (e == 1 || (b.n == 1 && c.read(b.p) < 2)) return b;
+    DN p = pow(c, b, e >> 1);
+    p = c.sqr(p);
+    if (e & 1) p = c.mul(p, b);
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Decimal output (natural.h:904-1039).  dec_base[i] = 10^(18 2^i); a piece
+// below dec_base[i+1] splits into hi = x / dec_base[i] and lo = x mod
+// dec_base[i] with the reference's reciprocal estimate and correction loop.
+// All pieces of one level are split together; pieces known to be zero
+// from the digit count (the reference's hi == 0 pruning, natural.h:978-981)
+// are skipped.
+// ---------------------------------------------------------------------------
+struct DecBases {
+    std::vector<DN> base, rec;
+    std::vector<u32> n_shl;
+};
+
+void dec_big_level(Ctx& c, const DN& cur, u64 sx, u64 j0, u64 count, const DN& base, const DN& rec, u32 n_shl,
+                   DN& nxt) {
+    const u64 w = base.n, xn = sx - (w - 1), rn = rec.n, pn = xn + rn, m = count - j0;
+    const u64 sbits = (w + 2) * 64 - n_shl;
+    DN P = c.alloc(m * pn);
+    std::vector<IReq> rq(m);
+    for (u64 j = 0; j < m; j++)
+        rq[j] = IReq{P.p + j * pn, cur.p + (j0 + j) * sx + (w - 1), xn, rec.p, rn, pn, nullptr, 0};
+    rcchk(dev_batch(rq.data(), m, c.st));
+    // hi -> nxt[2j], stride 2w between consecutive hi pieces
+    launch_shr_pieces(nxt.p + 2 * j0 * w, 2 * w, P.p, pn, pn, sbits, w, m, c.st);
+    DN Q = c.alloc(m * 2 * w);
+    for (u64 j = 0; j < m; j++)
+        rq[j] = IReq{Q.p + j * 2 * w, nxt.p + 2 * (j0 + j) * w, w, base.p, w, 2 * w, nullptr, 0};
+    rcchk(dev_batch(rq.data(), m, c.st));
+    // lo = x - hi * base over 2w words (borrow flags must stay 1)
+    DN lo = c.alloc(m * 2 * w);
+    DN fl = c.zeros((m + 1) / 2 + 1);
+    u32* flags = (u32*)fl.p;
+    std::vector<AddJob> jobs(m);
+    for (u64 j = 0; j < m; j++)
+        jobs[j] = AddJob{lo.p + j * 2 * w, cur.p + (j0 + j) * sx, Q.p + j * 2 * w, 2 * w, sx, 2 * w, 0, nullptr,
+                         flags + j, 1, 0};
+    c.run_adds(jobs);
+    std::vector<u32> hf(m);
+    hipchk(hipMemcpyAsync(hf.data(), flags, m * sizeof(u32), hipMemcpyDeviceToHost, c.st), "flags");
+    c.sync();
+    for (u64 j = 0; j < m; j++)
+        if (hf[j] != 1) raise(NNH_ERR_DOMAIN, "radix conversion: quotient estimate too large");
+    // correction: while lo >= base: hi += 1, lo -= base (natural.h:962-965)
+    DN cf = c.alloc((m + 7) / 8 + 1);
+    u8* cmpf = (u8*)cf.p;
+    std::vector<u8> hc(m);
+    for (int round = 0;; round++) {
+        launch_cmp_pieces(lo.p, 2 * w, 2 * w, base.p, w, m, cmpf, c.st);
+        hipchk(hipMemcpyAsync(hc.data(), cmpf, m, hipMemcpyDeviceToHost, c.st), "compare flags");
+        c.sync();
+        jobs.clear();
+        for (u64 j = 0; j < m; j++)
+            if (hc[j]) {
+                jobs.push_back(AddJob{lo.p + j * 2 * w, lo.p + j * 2 * w, base.p, 2 * w, 2 * w, w, 0, nullptr, nullptr,
+                                      1, 0});
+                u64* h = nxt.p + 2 * (j0 + j) * w;
+                jobs.push_back(AddJob{h, h, nullptr, w, w, 0, 0, nullptr, nullptr, 0, 1});
+            }
+        if (jobs.empty()) break;
+        if (round >= 8) raise(NNH_ERR_DOMAIN, "radix conversion: correction did not converge");
+        c.run_adds(jobs);
+    }
+    // lo -> nxt[2j+1]
+    launch_shr_pieces(nxt.p + (2 * j0 + 1) * w, 2 * w, lo.p, 2 * w, 2 * w, 0, w, m, c.st);
+}
+
+// Zero-padded digits of x below dec_base[I+1]: 18 * 2^(I+1) characters.
+std::string to_dec(Ctx& c, DN x) {
+    c.normalize(x);
+    DecBases B;
+    B.base.push_back(c.word(DEC_BASE0));
+    int I = -1;  // dec_init (natural.h:1013-1018)
+    while (c.cmp(x, B.base[I + 1]) >= 0) {
+        B.base.push_back(c.sqr(B.base[I + 1]));
+        I++;
+    }
+    if (I < 0) return std::to_string(c.read(x.p));
+    B.rec.resize(I + 1);
+    B.n_shl.assign(I + 1, 0);
+    for (int i = DEC_LEAF_LEVEL + 1; i <= I; i++) {  // calc_dec_base (natural.h:909-923)
+        u32 ns = (u32)((std::countl_zero(c.top(B.base[i])) + 1) & 63);
+        B.n_shl[i] = ns;
+        B.rec[i] = reciprocal_plain(c, c.shl(B.base[i], ns), B.base[i].n * 2 + 1);
+    }
+    const u64 total = 18ull << (I + 1);
+    // digits of x: at most floor(bits * log10 2) + 1
+    u64 bits = 64 * (x.n - 1) + (64 - std::countl_zero(c.top(x)));
+    u64 ndig = (u64)((double)bits * 0.30102999566398120) + 2;
+    if (ndig > total) ndig = total;
+    DN digits_dev = c.alloc((total + 7) / 8);
+    char* dd = (char*)digits_dev.p;
+    DN err = c.zeros(1);
+    int lvl = I;
+    u64 sx = B.base[I + 1].n;
+    DN cur = c.zeros(sx);
+    c.copy(cur.p, x.p, x.n);
+    auto zero_pieces = [&](int level) -> u64 {  // leading pieces of 18*2^(level+1) digits that are zero
+        u64 pd = 18ull << (level + 1);
+        return (total - ndig) / pd;
+    };
+    for (; lvl > DEC_LEAF_LEVEL; lvl--) {
+        const DN& base = B.base[lvl];
+        const u64 w = base.n;
+        const u64 count = 1ull << (I - lvl);
+        const u64 j0 = std::min(zero_pieces(lvl), count - 1);
+        DN nxt = c.alloc(2 * count * w);
+        if (j0) launch_fill(nxt.p, 2 * j0 * w, 0, c.st);
+        const DN& rec = B.rec[lvl];
+        if (w <= DEC_WG_MAX_WORDS && rec.n <= w + 16 && sx <= 2 * w) {
+            launch_dec_split_wg(cur.p + j0 * sx, sx, count - j0, base.p, w, rec.p, rec.n, B.n_shl[lvl],
+                                nxt.p + 2 * j0 * w, err.p, c.st);
+        } else {
+            dec_big_level(c, cur, sx, j0, count, base, rec, B.n_shl[lvl], nxt);
+        }
+        cur = nxt;
+        sx = w;
+    }
+    // leaves: pieces below dec_base[lvl+1] (lvl <= 3), one thread each
+    const u64 count = 1ull << (I - lvl);
+    const u64 j0 = std::min(zero_pieces(lvl), count - 1);
+    const u64 pd = 18ull << (lvl + 1);
+    if (sx > 16) raise(NNH_ERR_DOMAIN, "radix conversion: leaf pieces too long");
+    launch_dec_leaves(cur.p + j0 * sx, sx, count - j0, lvl, dd + j0 * pd, c.st);
+    if (c.read(err.p)) raise(NNH_ERR_DOMAIN, "radix conversion: quotient estimate out of range");
+    std::string s(total - j0 * pd, '0');
+    hipchk(hipMemcpyAsync(s.data(), dd + j0 * pd, s.size(), hipMemcpyDeviceToHost, c.st), "digits");
+    c.sync();
+    size_t nz = s.find_first_not_of('0');
+    if (nz == std::string::npos) return "0";
+    return s.substr(nz);
+}
+
+// ---------------------------------------------------------------------------
+// e by binary splitting over [0, 2^k), level-synchronous: level l holds
+// the 2^(k-l) ranges of 2^l terms at a common stride S(l) words.
+// ---------------------------------------------------------------------------
+u64 e_stride(int k, int l) {
+    // den of the last range [2^k - 2^l, 2^k) is (2^k - 1)! / (2^k - 2^l - 1)!,
+    // the largest of the level; num < 3 den.
+    long double e = std::ldexp(1.0L, k), b = e - std::ldexp(1.0L, l);
+    long double bits = (std::lgamma(e) - (b > 0 ? std::lgamma(b) : 0.0L)) / std::log(2.0L);
+    return (u64)std::ceil((double)((bits + 2.0L) / 64.0L)) + 1;
+}
+
+void calc_e(Ctx& c, int k, DN& num, DN& den) {
+    if (k < 6 || k > 26) raise(NNH_ERR_ARG, "calc_e: k must be in [6, 26]");
+    u64 S = e_stride(k, 5);
+    if (S > E_LEAF_MAX_WORDS) raise(NNH_ERR_ARG, "calc_e: leaf stride too large");
+    u64 nb = 1ull << (k - 5);
+    DN cn = c.alloc(nb * S), cd = c.alloc(nb * S);
+    launch_e_leaves(cn.p, cd.p, nb, S, c.st);
+    for (int l = 6; l <= k; l++) {
+        u64 S2 = e_stride(k, l), ncomb = 1ull << (k - l);
+        DN nn = c.alloc(ncomb * S2), nd = c.alloc(ncomb * S2);
+        if (S <= E_WG_MAX_WORDS && S2 <= 4096) {
+            launch_e_combine(cn.p, cd.p, S, nn.p, nd.p, S2, ncomb, c.st);
+        } else {
+            std::vector<IReq> rq(2 * ncomb);
+            for (u64 j = 0; j < ncomb; j++) {
+                const u64 L = 2 * j * S, R = (2 * j + 1) * S;
+                // natural.h:852-856: num = num * den1 + num1, den = den * den1
+                rq[2 * j] = IReq{nn.p + j * S2, cn.p + L, S, cd.p + R, S, S2, cn.p + R, S};
+                rq[2 * j + 1] = IReq{nd.p + j * S2, cd.p + L, S, cd.p + R, S, S2, nullptr, 0};
+            }
+            rcchk(dev_batch(rq.data(), rq.size(), c.st));
+        }
+        cn = nn;
+        cd = nd;
+        S = S2;
+    }
+    cn.n = S;
+    cd.n = S;
+    c.normalize(cn);
+    c.normalize(cd);
+    num = cn;
+    den = cd;
+}
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+
+template <class F>
+int guarded(F&& f) {
+    int rc = host_init();
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(host_mutex());
+    try {
+        Ctx c(host_stream());
+        f(c);
+        c.sync();
+        return NNH_OK;
+    } catch (const Err& e) {
+        (void)hipStreamSynchronize(host_stream());
+        return set_error(e.code, e.msg);
+    } catch (const std::bad_alloc&) {
+        return set_error(NNH_ERR_NOMEM, "host allocation failed");
+    }
+}
+
+u64 emit(Ctx& c, const DN& x, u64* out, u64 cap) {
+    if (x.n > cap) raise(NNH_ERR_ARG, "output buffer too small (" + std::to_string(x.n) + " words needed)");
+    c.download(out, x.p, x.n);
+    return x.n;
+}
+
+}  // namespace
+}  // namespace nnh
+
+using namespace nnh;
+
+extern "C" {
+
+int nnh_divmod(uint64_t* q, uint64_t q_cap, uint64_t* qn, uint64_t* r, uint64_t r_cap, uint64_t* rn,
+               const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn) {
+    if (!q || !r || !qn || !rn || !a || !b || !an || !bn) return set_error(NNH_ERR_ARG, "null pointer or empty operand");
+    return guarded([&](Ctx& c) {
+        DN da = c.upload(a, an), db = c.upload(b, bn);
+        c.normalize(da);
+        c.normalize(db);
+        DN dq, dr;
+        divmod(c, da, db, dq, dr);
+        c.normalize(dr);
+        *qn = emit(c, dq, q, q_cap);
+        *rn = emit(c, dr, r, r_cap);
+    });
+}
+
+int nnh_pow(uint64_t* dst, uint64_t cap, uint64_t* n, const uint64_t* b, uint64_t bn, uint64_t e) {
+    if (!dst || !n || !b || !bn) return set_error(NNH_ERR_ARG, "null pointer or empty operand");
+    return guarded([&](Ctx& c) {
+        DN db = c.upload(b, bn);
+        c.normalize(db);
+        DN p = pow(c, db, e);
+        *n = emit(c, p, dst, cap);
+    });
+}
+
+int nnh_to_dec(char* out, uint64_t cap, uint64_t* len, const uint64_t* a, uint64_t an) {
+    if (!out || !len || !a || !an) return set_error(NNH_ERR_ARG, "null pointer or empty operand");
+    return guarded([&](Ctx& c) {
+        DN da = c.upload(a, an);
+        std::string s = to_dec(c, da);
+        if (s.size() + 1 > cap) raise(NNH_ERR_ARG, "output buffer too small");
+        std::memcpy(out, s.data(), s.size());
+        out[s.size()] = 0;
+        *len = s.size();
+    });
+}
+
+int nnh_calc_e(uint64_t* num, uint64_t num_cap, uint64_t* num_n, uint64_t* den, uint64_t den_cap, uint64_t* den_n,
+               int k) {
+    if (!num || !den || !num_n || !den_n) return set_error(NNH_ERR_ARG, "null pointer");
+    return guarded([&](Ctx& c) {
+        DN dn, dd;
+        calc_e(c, k, dn, dd);
+        *num_n = emit(c, dn, num, num_cap);
+        *den_n = emit(c, dd, den, den_cap);
+    });
+}
+
+int nnh_e_digits(char* out, uint64_t cap, uint64_t* len, int k, uint64_t digits, double* stage_ms) {
+    if (!out || !len) return set_error(NNH_ERR_ARG, "null pointer");
+    return guarded([&](Ctx& c) {
+        double ms[NNH_E_STAGES] = {0, 0, 0, 0, 0};
+        auto t = Clock::now();
+        DN num, den;
+        calc_e(c, k, num, den);  // test.cpp:14
+        c.sync();
+        ms[0] = ms_since(t);
+        t = Clock::now();
+        DN p = pow(c, c.word(10), digits);  // test.cpp:15
+        c.sync();
+        ms[1] = ms_since(t);
+        t = Clock::now();
+        num = c.mul(num, p);
+        c.sync();
+        ms[2] = ms_since(t);
+        t = Clock::now();
+        DN q, r;
+        divmod(c, num, den, q, r);  // test.cpp:16
+        c.sync();
+        ms[3] = ms_since(t);
+        t = Clock::now();
+        std::string s = to_dec(c, q);  // test.cpp:18-19
+        ms[4] = ms_since(t);
+        if (s.size() + 1 > cap) raise(NNH_ERR_ARG, "output buffer too small");
+        std::memcpy(out, s.data(), s.size());
+        out[s.size()] = 0;
+        *len = s.size();
+        if (stage_ms)
+            for (int i = 0; i < NNH_E_STAGES; i++) stage_ms[i] = ms[i];
+    });
+}
+
+}  // extern "C"
