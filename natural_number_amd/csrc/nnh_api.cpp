@@ -78,6 +78,7 @@ struct Device {
     Buf tiles;                 // tile carry functions and carry-ins (bytes)
     Buf stage;                 // host-API staging: a, b, dst
     Buf tmp;                   // split addend / cyclic words / columns
+    Buf bws, bjobs, btiles;    // workspace of internal batches (reused across calls)
     std::mutex mu;
 };
 
@@ -599,20 +600,23 @@ struct nnh_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t cap_stream = nullptr;
+    bool borrowed = false;  // buffers belong to the device's internal batch workspace
 };
 
 namespace nnh {
 
 static void batch_free(nnh_batch* B) {
     if (!B) return;
-    (void)hipDeviceSynchronize();
+    if (!B->borrowed || B->exec) (void)hipDeviceSynchronize();
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
     if (B->cap_stream) (void)hipStreamDestroy(B->cap_stream);
-    if (B->d_jobs) (void)hipFree(B->d_jobs);
-    if (B->d_sj) (void)hipFree(B->d_sj);
-    if (B->d_tiles) (void)hipFree(B->d_tiles);
-    if (B->ws) (void)hipFree(B->ws);
+    if (!B->borrowed) {
+        if (B->d_jobs) (void)hipFree(B->d_jobs);
+        if (B->d_sj) (void)hipFree(B->d_sj);
+        if (B->d_tiles) (void)hipFree(B->d_tiles);
+        if (B->ws) (void)hipFree(B->ws);
+    }
     delete B;
 }
 
@@ -722,15 +726,29 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
         batch_free(B);
         return rc;
     }
-    auto alloc = [&](void** p, u64 bytes) -> int {
-        hipError_t e = hipMalloc(p, std::max<u64>(bytes, 64));
-        if (e != hipSuccess) return fail(NNH_ERR_NOMEM, std::string("batch allocation: ") + hipGetErrorString(e));
-        return NNH_OK;
-    };
-    if ((rc = alloc((void**)&B->ws, ws * 8)) || (rc = alloc((void**)&B->d_jobs, B->jobs.size() * sizeof(BJob))) ||
-        (rc = alloc((void**)&B->d_tiles, 2 * tiles)) || (rc = alloc((void**)&B->d_sj, B->n_school * 4 + 4))) {
-        batch_free(B);
-        return rc;
+    if (flags & 2) {
+        // internal batch: carve the buffers out of the device's reusable workspace
+        B->borrowed = true;
+        u64 jw = (B->jobs.size() * sizeof(BJob) + 7) / 8, tw = (2 * tiles + 7) / 8 + (B->n_school * 4 + 4 + 7) / 8 + 2;
+        if ((rc = ensure(D.bws, ws + 1)) || (rc = ensure(D.bjobs, jw + 1)) || (rc = ensure(D.btiles, tw))) {
+            batch_free(B);
+            return rc;
+        }
+        B->ws = D.bws.p;
+        B->d_jobs = (BJob*)D.bjobs.p;
+        B->d_tiles = (u8*)D.btiles.p;
+        B->d_sj = (u32*)(D.btiles.p + (2 * tiles + 7) / 8 + 1);
+    } else {
+        auto alloc = [&](void** p, u64 bytes) -> int {
+            hipError_t e = hipMalloc(p, std::max<u64>(bytes, 64));
+            if (e != hipSuccess) return fail(NNH_ERR_NOMEM, std::string("batch allocation: ") + hipGetErrorString(e));
+            return NNH_OK;
+        };
+        if ((rc = alloc((void**)&B->ws, ws * 8)) || (rc = alloc((void**)&B->d_jobs, B->jobs.size() * sizeof(BJob))) ||
+            (rc = alloc((void**)&B->d_tiles, 2 * tiles)) || (rc = alloc((void**)&B->d_sj, B->n_school * 4 + 4))) {
+            batch_free(B);
+            return rc;
+        }
     }
     for (BJob& J : B->jobs) {
         J.fa = B->ws + (uintptr_t)J.fa / 8;
@@ -794,7 +812,7 @@ int dev_mul_scale(u64* out, const u64* a, u64 an, const u64* b, u64 bn, int s, u
 int dev_batch(const IReq* reqs, u64 n, hipStream_t st) {
     if (!n) return NNH_OK;
     nnh_batch* B = nullptr;
-    int rc = batch_create(g_dev, reqs, n, 0, &B);
+    int rc = batch_create(g_dev, reqs, n, 2, &B);
     if (rc) return rc;
     batch_enqueue(g_dev, B, st);
     hipError_t e = hipGetLastError();
@@ -832,7 +850,7 @@ int nnh_shutdown(void) {
         g_dev.twf[i] = g_dev.twi[i] = nullptr;
     }
     g_dev.tw_scale = 0;
-    for (Buf* b : {&g_dev.ws, &g_dev.tiles, &g_dev.stage, &g_dev.tmp}) {
+    for (Buf* b : {&g_dev.ws, &g_dev.tiles, &g_dev.stage, &g_dev.tmp, &g_dev.bws, &g_dev.bjobs, &g_dev.btiles}) {
         if (b->p) (void)hipFree(b->p);
         *b = Buf{};
     }

@@ -8,7 +8,7 @@ template <int CB, bool FROM_LIMBS, bool SQR, bool FWD_ONLY>
 __global__ __launch_bounds__(1 << (CB - 4)) void k_mid(MidArgs A) {
     __shared__ u64 lds[(1 << CB) + (1 << (CB - 4))];
     const u32 pr = blockIdx.y;
-    mid_body<CB, FROM_LIMBS, SQR, FWD_ONLY>(A, pr, blockIdx.x, lds);
+    mid_body<CB, FROM_LIMBS, SQR, FWD_ONLY, false>(A, pr, blockIdx.x, lds);
 }
 
 
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(1 << (CB - 4)) void k_mid_batch(const BJob* __restr
         A.a_done = 0;
         A.b_done = 0;
         bool sq = J.sqr != 0;
-        mid_body<CB, FROM_LIMBS, false, false>(A, pr, chunk, lds, sq);
+        mid_body<CB, FROM_LIMBS, false, false, true>(A, pr, chunk, lds, sq);
         __syncthreads();
     }
 }

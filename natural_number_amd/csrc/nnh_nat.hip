@@ -519,6 +519,16 @@ __global__ void k_shr_pieces(u64* out, u64 so, const u64* in, u64 si, u64 n_in, 
     }
 }
 
+__global__ void k_nonzero_pieces(const u64* x, u64 sx, u64 lo, u64 len, u64 count, u64* flag) {
+    const u64 total = len * count;
+    bool nz = false;
+    for (u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (u64)gridDim.x * blockDim.x) {
+        const u64 j = g / len, k = g - j * len;
+        nz |= x[j * sx + lo + k] != 0;
+    }
+    if (__ballot(nz) && (threadIdx.x & 63) == 0) *flag = 1;
+}
+
 __global__ __launch_bounds__(256) void k_cmp_pieces(const u64* x, u64 sx, u64 nx, const u64* y, u64 ny, u64 count,
                                                     u8* flags) {
     __shared__ u32 sh;
@@ -543,6 +553,11 @@ void launch_shr_pieces(u64* out, u64 so, const u64* in, u64 si, u64 n_in, u64 sh
     if (total)
         hipLaunchKernelGGL(k_shr_pieces, dim3(grid_for(total)), dim3(256), 0, st, out, so, in, si, n_in, shift, n_out,
                            count);
+}
+void launch_nonzero_pieces(const u64* x, u64 sx, u64 lo, u64 hi, u64 count, u64* flag, hipStream_t st) {
+    if (hi > lo && count)
+        hipLaunchKernelGGL(k_nonzero_pieces, dim3(grid_for((hi - lo) * count)), dim3(256), 0, st, x, sx, lo, hi - lo,
+                           count, flag);
 }
 void launch_cmp_pieces(const u64* x, u64 sx, u64 nx, const u64* y, u64 ny, u64 count, u8* flags, hipStream_t st) {
     hipLaunchKernelGGL(k_cmp_pieces, dim3(persist_grid(count, 16384)), dim3(256), 0, st, x, sx, nx, y, ny, count,

@@ -66,6 +66,8 @@ void launch_dec_split_wg(const u64* x, u64 sx, u64 count, const u64* base, u64 w
 // out_j[0..n_out) = (in_j >> shift bits), pieces at strides so / si with n_in words.
 void launch_shr_pieces(u64* out, u64 so, const u64* in, u64 si, u64 n_in, u64 shift, u64 n_out, u64 count,
                        hipStream_t st);
+// *flag = 1 when any word [lo, hi) of any piece (stride sx) is nonzero.
+void launch_nonzero_pieces(const u64* x, u64 sx, u64 lo, u64 hi, u64 count, u64* flag, hipStream_t st);
 // flags[j] = (x_j >= y) for pieces x_j (stride sx, nx words) and one y (ny words).
 void launch_cmp_pieces(const u64* x, u64 sx, u64 nx, const u64* y, u64 ny, u64 count, u8* flags, hipStream_t st);
 

@@ -188,6 +188,19 @@ class Ctx {
         u64 h = find(x.p, nullptr, x.n, 0);
         x.n = h ? h : 1;
     }
+    // Normalise a product or sum of normalised operands: its value is at
+    // least B^(n-2) unless an operand was zero, so the top two words decide.
+    void normalize_top(DN& x) {
+        if (x.n < 2) return;
+        hipchk(hipMemcpyAsync(h_, x.p + x.n - 2, 2 * sizeof(u64), hipMemcpyDeviceToHost, st), "read");
+        sync();
+        if (h_[1]) return;
+        if (h_[0]) {
+            x.n -= 1;
+            return;
+        }
+        normalize(x);
+    }
     bool is_zero(const DN& x) { return x.n == 1 && read(x.p) == 0; }
     // Compare normalised naturals (natural.h:145-160).
     int cmp(const DN& a, const DN& b) {
@@ -249,7 +262,7 @@ class Ctx {
         std::vector<AddJob> j(1);
         j[0] = AddJob{d.p, a.p, b.p, a.n + 1, a.n, b.n, 0, nullptr, nullptr, 0, 0};
         run_adds(j);
-        normalize(d);
+        normalize_top(d);
         return d;
     }
     // natural.h:324-342 for x >= y
@@ -296,16 +309,17 @@ class Ctx {
     }
 
     // ---- products ------------------------------------------------------------
+    // Products of normalised operands (every caller normalises its inputs).
     DN mul(const DN& a, const DN& b) {
         DN d = alloc(a.n + b.n);
         rcchk(dev_mul(d.p, a.p, a.n, b.p, b.n, st));
-        normalize(d);
+        normalize_top(d);
         return d;
     }
     DN sqr(const DN& a) {
         DN d = alloc(2 * a.n);
         rcchk(dev_sqr(d.p, a.p, a.n, st));
-        normalize(d);
+        normalize_top(d);
         return d;
     }
     // NTT_info load/mul/INTT/save at scale s (ntt_info.h:147-248).
@@ -315,7 +329,8 @@ class Ctx {
         u64 n = 0;
         rcchk(dev_mul_scale(d.p, a.p, a.n, b.p, b.n, s, save_size, &n, st));
         d.n = n;
-        normalize(d);
+        if (n == a.n + b.n) normalize_top(d);  // exact product
+        else normalize(d);                     // residue mod B^L - 1
         return d;
     }
     // mul_save_NTT_info (natural.h:452-467): d = a * b, x keeps a's transform.
@@ -572,12 +587,17 @@ void dec_big_level(Ctx& c, const DN& cur, u64 sx, u64 j0, u64 count, const DN& b
     c.sync();
     for (u64 j = 0; j < m; j++)
         if (hf[j] != 1) raise(NNH_ERR_DOMAIN, "radix conversion: quotient estimate too large");
+    // lo < 3 base < B^(w+1) when the estimate is within 2 of the quotient
+    // (it is for the reference's reciprocal); words w+1 .. 2w-1 must be zero.
+    DN zf = c.zeros(1);
+    launch_nonzero_pieces(lo.p, 2 * w, w + 1, 2 * w, m, zf.p, c.st);
+    if (c.read(zf.p)) raise(NNH_ERR_DOMAIN, "radix conversion: quotient estimate too small");
     // correction: while lo >= base: hi += 1, lo -= base (natural.h:962-965)
     DN cf = c.alloc((m + 7) / 8 + 1);
     u8* cmpf = (u8*)cf.p;
     std::vector<u8> hc(m);
     for (int round = 0;; round++) {
-        launch_cmp_pieces(lo.p, 2 * w, 2 * w, base.p, w, m, cmpf, c.st);
+        launch_cmp_pieces(lo.p, 2 * w, w + 1, base.p, w, m, cmpf, c.st);
         hipchk(hipMemcpyAsync(hc.data(), cmpf, m, hipMemcpyDeviceToHost, c.st), "compare flags");
         c.sync();
         jobs.clear();

@@ -113,6 +113,13 @@ __device__ __forceinline__ void inv_rec(const Prime& P, u64 (&v)[16], u64* lds, 
     }
 }
 
+// Element i of a three-entry (per prime) array without dynamic indexing,
+// which would force the argument struct into scratch memory.
+template <class T>
+__device__ __forceinline__ T sel3(const T (&a)[3], u32 i) {
+    return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]);
+}
+
 // ---------------------------------------------------------------------------
 // Operand load: residue of limb x scaled by alpha (Montgomery form alpha*R),
 // zero beyond the operand.  Reference ntt_info.h:147-173 reduces by Barrett;
@@ -150,7 +157,9 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
     const u64 blk = tile / tiles_per_blk;
     const u64 cbase = (tile % tiles_per_blk) << Q;
     const u64 xbase = (blk << (A.g_lo + NLEV)) + cbase;
-    u64* __restrict__ data = A.data[pr];
+    u64* __restrict__ data = sel3(A.data, pr);
+    const u64* __restrict__ tw = sel3(A.tw, pr);
+    const u64 alpha = sel3(A.alpha_m, pr);
     u64 v[16];
     {
         const u32 yt = lay_t(t, LO_IO);
@@ -158,15 +167,15 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
             constexpr u32 jy = (u32)decltype(jc)::value << LO_IO;
             const u32 y = yt | jy;
             const u64 x = xbase + (y & QM) + (u64)(y >> Q) * d;
-            if constexpr (FROM_LIMBS) v[decltype(jc)::value] = load_limb(P, A.limbs, A.an, x, A.alpha_m[pr]);
+            if constexpr (FROM_LIMBS) v[decltype(jc)::value] = load_limb(P, A.limbs, A.an, x, alpha);
             else v[decltype(jc)::value] = data[x];
         });
     }
     if constexpr (!INV) {
-        fwd_rec<CB, CB - 1, Q, CB - 1, LO_IO, Q>(P, v, lds, t, blk, A.tw[pr]);
+        fwd_rec<CB, CB - 1, Q, CB - 1, LO_IO, Q>(P, v, lds, t, blk, tw);
     } else {
         if constexpr (LO_INV0 != LO_IO) xchg<CB, LO_IO, LO_INV0>(v, lds, t);
-        inv_rec<CB, Q, Q, CB - 1, LO_INV0>(P, v, lds, t, blk, A.tw[pr]);
+        inv_rec<CB, Q, Q, CB - 1, LO_INV0>(P, v, lds, t, blk, tw);
     }
     if constexpr (LO_ST != LO_END) xchg<CB, LO_END, LO_ST>(v, lds, t);
     {
@@ -213,7 +222,10 @@ struct MidArgs {
     int a_done, b_done;
 };
 
-template <int CB, bool FROM_LIMBS, bool SQR, bool FWD_ONLY>
+// SEL: pick the per-prime pointers with selects (an argument struct built
+// in registers, batched kernels) instead of indexing the kernel-argument
+// struct directly (single-product kernel, where that keeps fewer registers).
+template <int CB, bool FROM_LIMBS, bool SQR, bool FWD_ONLY, bool SEL>
 __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u64* lds, bool sqr_rt = false) {
     const Prime P = make_prime(PRIME_K[0] * (pr == 0) + PRIME_K[1] * (pr == 1) + PRIME_K[2] * (pr == 2));
     constexpr int LO_IO = CB - 4;
@@ -221,24 +233,36 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
     const u32 t = threadIdx.x;
     const u64 base = chunk << CB;
     const u32 yt = lay_t(t, LO_IO);
+    // SEL: selected once; otherwise indexed at each use, which lets the
+    // compiler rematerialise the kernel-argument loads
+    u64* const fa_s = SEL ? sel3(A.fa, pr) : nullptr;
+    const u64* const fb_s = SEL ? sel3(A.fb, pr) : nullptr;
+    const u64* const twf_s = SEL ? sel3(A.twf, pr) : nullptr;
+    const u64* const twi_s = SEL ? sel3(A.twi, pr) : nullptr;
+    const u64 alpha_s = SEL ? sel3(A.alpha_m, pr) : 0;
+    auto fa_ = [&]() -> u64* { if constexpr (SEL) return fa_s; else return A.fa[pr]; };
+    auto fb_ = [&]() -> const u64* { if constexpr (SEL) return fb_s; else return A.fb[pr]; };
+    auto twf_ = [&]() -> const u64* { if constexpr (SEL) return twf_s; else return A.twf[pr]; };
+    auto twi_ = [&]() -> const u64* { if constexpr (SEL) return twi_s; else return A.twi[pr]; };
+    auto alpha_ = [&]() -> u64 { if constexpr (SEL) return alpha_s; else return A.alpha_m[pr]; };
     u64 a[16], b[16];
     CFor<0, 16>::run([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         const u64 x = base + (yt | ((u32)j << LO_IO));
-        if constexpr (FROM_LIMBS) a[j] = load_limb(P, A.la, A.an, x, A.alpha_m[pr]);
-        else a[j] = A.fa[pr][x];
+        if constexpr (FROM_LIMBS) a[j] = load_limb(P, A.la, A.an, x, alpha_());
+        else a[j] = fa_()[x];
     });
     if constexpr (FWD_ONLY) {
-        fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0>(P, a, lds, t, chunk, A.twf[pr]);
+        fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0>(P, a, lds, t, chunk, twf_());
         if constexpr (LO_F != LO_IO) xchg<CB, LO_F, LO_IO>(a, lds, t);
         CFor<0, 16>::run([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            A.fa[pr][base + (yt | ((u32)j << LO_IO))] = a[j];
+            fa_()[base + (yt | ((u32)j << LO_IO))] = a[j];
         });
         return;
     } else {
         if (!A.a_done) {
-            fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0>(P, a, lds, t, chunk, A.twf[pr]);
+            fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0>(P, a, lds, t, chunk, twf_());
             if constexpr (LO_F != 0) xchg<CB, LO_F, 0>(a, lds, t);
         } else {
             xchg<CB, LO_IO, 0>(a, lds, t);
@@ -247,11 +271,11 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
             CFor<0, 16>::run([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
                 const u64 x = base + (yt | ((u32)j << LO_IO));
-                if (A.lb) b[j] = load_limb(P, A.lb, A.bn, x, A.alpha_m[pr]);
-                else b[j] = A.fb[pr][x];
+                if (A.lb) b[j] = load_limb(P, A.lb, A.bn, x, alpha_());
+                else b[j] = fb_()[x];
             });
             if (!A.b_done) {
-                fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0>(P, b, lds, t, chunk, A.twf[pr]);
+                fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0>(P, b, lds, t, chunk, twf_());
                 if constexpr (LO_F != 0) xchg<CB, LO_F, 0>(b, lds, t);
             } else {
                 xchg<CB, LO_IO, 0>(b, lds, t);
@@ -260,17 +284,17 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
             CFor<0, 16>::run([&](auto jc) { b[decltype(jc)::value] = a[decltype(jc)::value]; });
         }
         // base multiply: window 0 holds y = 16t + j, groups g = 4t + j/4
-        const u64* twg = A.twf[pr] + (base >> 2) + 4 * (u64)t;
+        const u64* twg = twf_() + (base >> 2) + 4 * (u64)t;
         CFor<0, 4>::run([&](auto gc) {
             constexpr int g = decltype(gc)::value, j = 4 * g;
             base_mul4(P, a[j], a[j + 1], a[j + 2], a[j + 3], b[j], b[j + 1], b[j + 2], b[j + 3], twg[g]);
         });
-        inv_rec<CB, 1, 1, CB - 1, 0>(P, a, lds, t, chunk, A.twi[pr]);
+        inv_rec<CB, 1, 1, CB - 1, 0>(P, a, lds, t, chunk, twi_());
         constexpr int LO_I = inv_window(CB - 1, 1, 0, CB);
         if constexpr (LO_I != LO_IO) xchg<CB, LO_I, LO_IO>(a, lds, t);
         CFor<0, 16>::run([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            A.fa[pr][base + (yt | ((u32)j << LO_IO))] = a[j];
+            fa_()[base + (yt | ((u32)j << LO_IO))] = a[j];
         });
     }
 }

@@ -5,17 +5,16 @@
 
 namespace nnh {
 
-template <bool INV, bool FROM_LIMBS>
+template <bool INV, bool FROM_LIMBS, int NLEV>
 __global__ __launch_bounds__(256) void k_top_pass(TopArgs A) {
     __shared__ u64 lds[4096 + 256];
-    const u32 pr = blockIdx.y;
-    top_pass_nlev<INV, FROM_LIMBS>(A, pr, blockIdx.x, lds);
+    top_pass_body<INV, FROM_LIMBS, NLEV>(A, blockIdx.y, blockIdx.x, lds);
 }
 
 
 // Batched: persistent grid-stride loop over (job, tile, prime) items of a
 // bucket of equal transform length.
-template <bool INV, bool FROM_LIMBS>
+template <bool INV, bool FROM_LIMBS, int NLEV>
 __global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__ jobs, u32 njobs, int operand,
                                                         TopBatchParams P) {
     __shared__ u64 lds[4096 + 256];
@@ -38,11 +37,36 @@ __global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__
         A.an = operand ? J.bn : J.an;
         A.g_lo = P.g_lo;
         A.nlev = P.nlev;
-        top_pass_nlev<INV, FROM_LIMBS>(A, pr, tile, lds);
+        top_pass_body<INV, FROM_LIMBS, NLEV>(A, pr, tile, lds);
         __syncthreads();
     }
 }
 
+
+// Host dispatch on the pass depth (1..8 levels), known per launch.
+template <bool INV, bool FL>
+static void top_single(const TopArgs& A, u32 nlev, dim3 grid, hipStream_t st) {
+    switch (nlev) {
+#define NNH_TP(n) \
+    case n: hipLaunchKernelGGL((k_top_pass<INV, FL, n>), grid, dim3(256), 0, st, A); break;
+        NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
+        default: hipLaunchKernelGGL((k_top_pass<INV, FL, 8>), grid, dim3(256), 0, st, A); break;
+#undef NNH_TP
+    }
+}
+template <bool INV, bool FL>
+static void top_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, dim3 grid, hipStream_t st) {
+    switch (P.nlev) {
+#define NNH_TP(n)                                                                                              \
+    case n:                                                                                                    \
+        hipLaunchKernelGGL((k_top_pass_batch<INV, FL, n>), grid, dim3(256), 0, st, jobs, njobs, operand, P); \
+        break;
+        NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
+        default: hipLaunchKernelGGL((k_top_pass_batch<INV, FL, 8>), grid, dim3(256), 0, st, jobs, njobs, operand, P);
+            break;
+#undef NNH_TP
+    }
+}
 
 void launch_top_pass(const TopPassDesc& D, hipStream_t st) {
     TopArgs A;
@@ -55,19 +79,19 @@ void launch_top_pass(const TopPassDesc& D, hipStream_t st) {
     A.an = D.an;
     A.g_lo = D.g_lo;
     A.nlev = D.nlev;
-    dim3 grid((unsigned)(D.L >> 12), 3), block(256);
-    if (D.inverse) hipLaunchKernelGGL((k_top_pass<true, false>), grid, block, 0, st, A);
-    else if (D.limbs) hipLaunchKernelGGL((k_top_pass<false, true>), grid, block, 0, st, A);
-    else hipLaunchKernelGGL((k_top_pass<false, false>), grid, block, 0, st, A);
+    dim3 grid((unsigned)(D.L >> 12), 3);
+    if (D.inverse) top_single<true, false>(A, D.nlev, grid, st);
+    else if (D.limbs) top_single<false, true>(A, D.nlev, grid, st);
+    else top_single<false, false>(A, D.nlev, grid, st);
 }
 
 void launch_top_pass_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
                            bool from_limbs, hipStream_t st) {
     u64 items = (u64)njobs * (P.L >> 12) * 3;
-    dim3 grid(persist_grid(items, 4096)), block(256);
-    if (inverse) hipLaunchKernelGGL((k_top_pass_batch<true, false>), grid, block, 0, st, jobs, njobs, operand, P);
-    else if (from_limbs) hipLaunchKernelGGL((k_top_pass_batch<false, true>), grid, block, 0, st, jobs, njobs, operand, P);
-    else hipLaunchKernelGGL((k_top_pass_batch<false, false>), grid, block, 0, st, jobs, njobs, operand, P);
+    dim3 grid(persist_grid(items, 4096));
+    if (inverse) top_batch<true, false>(jobs, njobs, operand, P, grid, st);
+    else if (from_limbs) top_batch<false, true>(jobs, njobs, operand, P, grid, st);
+    else top_batch<false, false>(jobs, njobs, operand, P, grid, st);
 }
 
 }  // namespace nnh
