@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "bit-exact product throughput: bits/s at 2^30-bit NxN; batched mix products/s"
 HBM_PEAK = 8.0e12
+# HBM bytes per launch of the fused chunk kernel at this workload: rocprofv3
+# FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the counter's reading
+# for a plain copy of known size (profiles/ntt_product_2e30x2e30_pmc.md).
+MID_FUSED_TRAFFIC = 3.429e9
 
 
 def log(*a):
@@ -104,6 +108,44 @@ def batch_mix(nn, stream, reps=5):
     }
 
 
+def e_workload(nn, with_cpu):
+    """Config 4: e by binary splitting (reference test.cpp:12-19): calc_e over
+    [0, 2^k), times 10^D, the reference's Newton division, decimal output.
+    GPU wall time per digit count, digits checked against the committed
+    digests; the oracle (one thread) is timed on the 10^7-digit case."""
+    with open(os.path.join(ROOT, "tests", "golden", "e_digits.json")) as f:
+        golden = json.load(f)
+    log("e: warm-up (10^6 digits)")
+    nn.e_digits(18, 10**6)
+    res = {}
+    for name in ("e_k21_D10000000", "e_k24_D100000000"):
+        g = golden[name]
+        log(f"e: {g['digits']} digits on the GPU")
+        t = time.perf_counter()
+        s, stages = nn.e_digits(g["k"], g["digits"])
+        wall = time.perf_counter() - t
+        res[name] = {
+            "k": g["k"], "digits": g["digits"], "gpu_wall_s": round(wall, 4),
+            "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+            "digits_match_reference": hashlib.sha256(s.encode()).hexdigest() == g["sha256"],
+        }
+        del s
+    if with_cpu:
+        g = golden["e_k21_D10000000"]
+        log("e: oracle on 10^7 digits (1 thread)")
+        lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+        lib.nno_e_digits.restype = ctypes.c_uint64
+        lib.nno_e_digits.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64]
+        buf = ctypes.create_string_buffer(g["digits"] + 64)
+        t = time.perf_counter()
+        n = lib.nno_e_digits(buf, g["digits"] + 64, g["k"], g["digits"])
+        tcpu = time.perf_counter() - t
+        ok = hashlib.sha256(buf.raw[:n]).hexdigest() == g["sha256"]
+        res["cpu_oracle_e_k21_D10000000"] = {"seconds": round(tcpu, 3), "cores": 1, "digits_match_reference": ok}
+        res["speedup_vs_cpu_at_1e7_digits"] = round(tcpu / res["e_k21_D10000000"]["gpu_wall_s"], 1)
+    return res
+
+
 def cost_model_run(n_side, n_held):
     """Config 5: timings over a log-uniform (m, n) grid of 2^10 .. 2^30-bit
     operands and batch counts, the FP32 MLP trained on the GPU by the
@@ -163,6 +205,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-mix measurement")
+    ap.add_argument("--no-e", action="store_true", help="skip the e-digits workload")
     ap.add_argument("--cost-model", action="store_true",
                     help="only run the cost-model pipeline (grid timings, on-device training, held-out regret)")
     ap.add_argument("--held-out", type=int, default=10000, help="held-out requests for --cost-model")
@@ -252,6 +295,7 @@ def main():
     b_alg = 16 * (an + bn) + 144 * L
 
     batch = None if args.no_batch else batch_mix(nn, stream)
+    e_res = None if args.no_e else e_workload(nn, not args.no_cpu)
 
     res = out.download()
     gpu_digest = hashlib.sha256(res.tobytes()).hexdigest()
@@ -299,7 +343,7 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": MID_FUSED_TRAFFIC if dom == "mid_fused" else None,
         },
         "product_roofline": {
             "alg_bytes": b_alg,
@@ -309,6 +353,7 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
         "batch_mix": batch,
+        "e_digits": e_res,
         "bit_exact_vs_cpu": exact,
         "result_sha256": gpu_digest,
         "wall_s_timed": wall,

@@ -25,12 +25,13 @@ __device__ __forceinline__ u32 find_add_job(const AddJob* jobs, u32 njobs, u64 t
     return lo;
 }
 
-__global__ __launch_bounds__(256) void k_add_tiles(const AddJob* __restrict__ jobs, u32 njobs, u64 ntiles,
+// jobs == nullptr: the single job `one` passed by value.
+__global__ __launch_bounds__(256) void k_add_tiles(const AddJob* __restrict__ jobs, AddJob one, u32 njobs, u64 ntiles,
                                                    u8* tile_fn) {
     __shared__ u32 wco[2][4];
     const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
     for (u64 it = blockIdx.x; it < ntiles; it += gridDim.x) {
-        const AddJob J = jobs[find_add_job(jobs, njobs, it)];
+        const AddJob J = jobs ? jobs[find_add_job(jobs, njobs, it)] : one;
         const u64 tile_base = (it - J.tile0) * LIMB_TILE;
         u32 cin0 = 0, cin1 = 1;
         for (int r = 0; r < CRT_TILE_ROUNDS; r++) {
@@ -70,11 +71,11 @@ __global__ __launch_bounds__(256) void k_add_tiles(const AddJob* __restrict__ jo
     }
 }
 
-__global__ __launch_bounds__(1024) void k_add_scan(const AddJob* __restrict__ jobs, u32 njobs, u64 ntiles,
-                                                   const u8* tile_fn, u8* tile_cin) {
+__global__ __launch_bounds__(1024) void k_add_scan(const AddJob* __restrict__ jobs, AddJob one, u32 njobs,
+                                                   u64 ntiles, const u8* tile_fn, u8* tile_cin) {
     __shared__ u32 chunk_fn[1024];
     for (u32 j = blockIdx.x; j < njobs; j += gridDim.x) {
-        const AddJob& J = jobs[j];
+        const AddJob J = jobs ? jobs[j] : one;
         const u64 t0 = J.tile0, t1 = j + 1 < njobs ? jobs[j + 1].tile0 : ntiles;
         const u32 cin = J.sub ? 1u : (J.cin_ptr ? *J.cin_ptr : J.cin);
         __syncthreads();
@@ -82,12 +83,12 @@ __global__ __launch_bounds__(1024) void k_add_scan(const AddJob* __restrict__ jo
     }
 }
 
-__global__ __launch_bounds__(256) void k_add_fix(const AddJob* __restrict__ jobs, u32 njobs, u64 ntiles,
+__global__ __launch_bounds__(256) void k_add_fix(const AddJob* __restrict__ jobs, AddJob one, u32 njobs, u64 ntiles,
                                                  const u8* tile_cin) {
     __shared__ u32 first_nz;
     for (u64 it = blockIdx.x; it < ntiles; it += gridDim.x) {
         if (!tile_cin[it]) continue;  // uniform
-        const AddJob& J = jobs[find_add_job(jobs, njobs, it)];
+        const AddJob J = jobs ? jobs[find_add_job(jobs, njobs, it)] : one;
         __syncthreads();
         carry_fix_body(J.out, J.n, it - J.tile0, first_nz);
     }
@@ -99,13 +100,21 @@ u64 add_tiles_needed(const AddJob* jobs, u64 njobs) {
     return t;
 }
 
+static void add_launch(const AddJob* d_jobs, const AddJob& one, u32 njobs, u64 ntiles, u8* d_tiles,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_add_tiles, dim3(persist_grid(ntiles, 8192)), dim3(256), 0, st, d_jobs, one, njobs, ntiles,
+                       d_tiles);
+    hipLaunchKernelGGL(k_add_scan, dim3(persist_grid(njobs, 1024)), dim3(1024), 0, st, d_jobs, one, njobs, ntiles,
+                       (const u8*)d_tiles, d_tiles + ntiles);
+    hipLaunchKernelGGL(k_add_fix, dim3(persist_grid(ntiles, 8192)), dim3(256), 0, st, d_jobs, one, njobs, ntiles,
+                       (const u8*)(d_tiles + ntiles));
+}
 void launch_add_batch(AddJob* d_jobs, u32 njobs, u64 ntiles, u8* d_tiles, hipStream_t st) {
-    hipLaunchKernelGGL(k_add_tiles, dim3(persist_grid(ntiles, 8192)), dim3(256), 0, st, (const AddJob*)d_jobs, njobs,
-                       ntiles, d_tiles);
-    hipLaunchKernelGGL(k_add_scan, dim3(persist_grid(njobs, 1024)), dim3(1024), 0, st, (const AddJob*)d_jobs, njobs,
-                       ntiles, (const u8*)d_tiles, d_tiles + ntiles);
-    hipLaunchKernelGGL(k_add_fix, dim3(persist_grid(ntiles, 8192)), dim3(256), 0, st, (const AddJob*)d_jobs, njobs,
-                       ntiles, (const u8*)(d_tiles + ntiles));
+    add_launch(d_jobs, AddJob{}, njobs, ntiles, d_tiles, st);
+}
+void launch_add_one(AddJob job, u8* d_tiles, hipStream_t st) {
+    job.tile0 = 0;
+    add_launch(nullptr, job, 1, (job.n + LIMB_TILE - 1) / LIMB_TILE, d_tiles, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -27,6 +27,8 @@ constexpr u64 LIMB_TILE = 4096;
 // least njobs AddJobs and 2*ntiles bytes (ntiles = sum ceil(n/4096)).
 u64 add_tiles_needed(const AddJob* jobs, u64 njobs);
 void launch_add_batch(AddJob* d_jobs, u32 njobs, u64 ntiles, u8* d_tiles, hipStream_t st);
+// One addition passed by value (no job table); d_tiles holds 2*ceil(n/4096) bytes.
+void launch_add_one(AddJob job, u8* d_tiles, hipStream_t st);
 
 void launch_not(u64* out, const u64* a, u64 n, hipStream_t st);
 void launch_fill(u64* out, u64 n, u64 v, hipStream_t st);

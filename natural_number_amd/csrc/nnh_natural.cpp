@@ -15,6 +15,8 @@
 #include <bit>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -39,6 +41,11 @@ constexpr int E_WG_MAX_WORDS = 256;      // e levels combined by one work-group
 constexpr int E_LEAF_MAX_WORDS = 14;
 
 int ntt_scale(u64 s1, u64 s2) { return 64 - std::countl_zero(s1 + s2 - 2); }  // natural.h:384-386
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+// NNH_TRACE=1 prints phase timings of the long computations to stderr.
+const bool g_trace = std::getenv("NNH_TRACE") != nullptr;
 
 struct Err {
     int code;
@@ -179,27 +186,29 @@ class Ctx {
         if (wb) *wb = h_[2];
         return h_[0];
     }
-    // natural.h:82-93
-    void normalize(DN& x) {
+    // natural.h:82-93; returns the top word.
+    u64 normalize(DN& x) {
         if (x.n == 0) {
             x = word(0);
-            return;
+            return 0;
         }
-        u64 h = find(x.p, nullptr, x.n, 0);
+        u64 w = 0;
+        u64 h = find(x.p, nullptr, x.n, 0, &w);
         x.n = h ? h : 1;
+        return w;
     }
     // Normalise a product or sum of normalised operands: its value is at
     // least B^(n-2) unless an operand was zero, so the top two words decide.
-    void normalize_top(DN& x) {
-        if (x.n < 2) return;
+    u64 normalize_top(DN& x) {
+        if (x.n < 2) return read(x.p);
         hipchk(hipMemcpyAsync(h_, x.p + x.n - 2, 2 * sizeof(u64), hipMemcpyDeviceToHost, st), "read");
         sync();
-        if (h_[1]) return;
+        if (h_[1]) return h_[1];
         if (h_[0]) {
             x.n -= 1;
-            return;
+            return h_[0];
         }
-        normalize(x);
+        return normalize(x);
     }
     bool is_zero(const DN& x) { return x.n == 1 && read(x.p) == 0; }
     // Compare normalised naturals (natural.h:145-160).
@@ -239,38 +248,41 @@ class Ctx {
         launch_add_batch(d_jobs_, (u32)jobs.size(), tiles, d_tiles_, st);
         sync();  // the staging buffer is reused by the next call
     }
+    // One addition enqueued with its job by value (no host synchronisation).
+    void add_one(const AddJob& j) {
+        u64 need = 2 * ((j.n + LIMB_TILE - 1) / LIMB_TILE);
+        if (need > tiles_cap_) {
+            sync();
+            if (d_tiles_) (void)hipFree(d_tiles_);
+            tiles_cap_ = std::bit_ceil(std::max<u64>(need, 4096));
+            hipchk(hipMalloc((void**)&d_tiles_, tiles_cap_), "tile flags");
+        }
+        launch_add_one(j, d_tiles_, st);
+    }
     // dst[0..as) = a + b (as >= bs), the carry dropped or stored to *cout.
     void add_to(u64* dst, const u64* a, u64 as, const u64* b, u64 bs, u32* cout = nullptr,
                 const u32* cin_ptr = nullptr) {
         if (bs > as) raise(NNH_ERR_DOMAIN, "addition with a longer second operand");
-        std::vector<AddJob> j(1);
-        j[0] = AddJob{dst, a, b, as, as, bs, 0, cin_ptr, cout, 0, 0};
-        run_adds(j);
+        add_one(AddJob{dst, a, b, as, as, bs, 0, cin_ptr, cout, 0, 0});
     }
     // dst[0..as) = a - b (as >= bs), the borrow dropped.
     void sub_from(u64* dst, const u64* a, u64 as, const u64* b, u64 bs) {
         if (bs > as) raise(NNH_ERR_DOMAIN, "subtraction with a longer second operand");
-        std::vector<AddJob> j(1);
-        j[0] = AddJob{dst, a, b, as, as, bs, 0, nullptr, nullptr, 1, 0};
-        run_adds(j);
+        add_one(AddJob{dst, a, b, as, as, bs, 0, nullptr, nullptr, 1, 0});
     }
     // natural.h:303-311
     DN add(const DN& x, const DN& y) {
         const DN& a = x.n >= y.n ? x : y;
         const DN& b = x.n >= y.n ? y : x;
         DN d = alloc(a.n + 1);
-        std::vector<AddJob> j(1);
-        j[0] = AddJob{d.p, a.p, b.p, a.n + 1, a.n, b.n, 0, nullptr, nullptr, 0, 0};
-        run_adds(j);
+        add_one(AddJob{d.p, a.p, b.p, a.n + 1, a.n, b.n, 0, nullptr, nullptr, 0, 0});
         normalize_top(d);
         return d;
     }
     // natural.h:324-342 for x >= y
     DN sub(const DN& x, const DN& y) {
         DN d = alloc(x.n);
-        std::vector<AddJob> j(1);
-        j[0] = AddJob{d.p, x.p, y.p, x.n, x.n, y.n, 0, nullptr, nullptr, 1, 0};
-        run_adds(j);
+        add_one(AddJob{d.p, x.p, y.p, x.n, x.n, y.n, 0, nullptr, nullptr, 1, 0});
         normalize(d);
         return d;
     }
@@ -387,8 +399,7 @@ DN reciprocal_plain(Ctx& c, const DN& divs, u64 divd_size) {
         tmp = c.mul_save(xa, ans, tmp);
         if (tmp.n <= cur) raise(NNH_ERR_DOMAIN, "reciprocal: product shorter than the estimate");
         tmp = c.complement(tmp.p + cur, tmp.n - cur);
-        c.normalize(tmp);
-        int lz = std::countl_zero(c.top(tmp));
+        int lz = std::countl_zero(c.normalize(tmp));
         u64 nxt = std::min(2 * cur, (2 * cur - tmp.n) * 2 + (u64)((lz + 31) >> 5));
         if (nxt == NTT_DIV_THRESHOLD) nxt = NTT_DIV_THRESHOLD - 1;
         if (nxt <= cur)  // no progress: the reference would shift by a negative count
@@ -422,11 +433,13 @@ DN reciprocal_ntt(Ctx& c, const DN& divisor, u64 dividend_size, Xf& xdiv) {
             tmp = c.mul_scale(ans, seg, s, seg_size + prev);
         }
         u64 n_shl = seg_size - 1 - prev;
+        u64 topw = 0, h = 0;
         if (tmp.n > n_shl + 1) {
-            u64 h = c.find(tmp.p + n_shl + 1, nullptr, tmp.n - n_shl - 1, 1);
+            h = c.find(tmp.p + n_shl + 1, nullptr, tmp.n - n_shl - 1, 1, &topw);
             tmp.n = n_shl + 1 + h;
         }
-        int lz = std::countl_zero(~c.top(tmp));
+        if (!h) topw = c.top(tmp);
+        int lz = std::countl_zero(~topw);
         u64 cur = std::min(2 * prev, (2 * prev - tmp.n + n_shl) * 2 + (u64)((lz + 31) >> 5));
         if (cur == NTT_DIV_THRESHOLD) cur -= 1;
         if (cur < prev)  // the reference's natural.h:675-677 then writes ~2^64 words
@@ -462,7 +475,13 @@ void div_iterative(Ctx& c, const DN& divd, const DN& divs, DN& q, DN& r) {
     DN b = c.shl(divs, (u64)n_shl);
     r = c.shl(divd, (u64)n_shl);
     Xf xb;
+    auto t0 = Clock::now();
     DN rec = reciprocal_ntt(c, b, r.n, xb);
+    if (g_trace) {
+        c.sync();
+        std::fprintf(stderr, "[nnh] division reciprocal (%llu words)   %9.3f ms\n", (unsigned long long)rec.n,
+                     ms_since(t0));
+    }
     Xf xrec;
     xrec.scale = ntt_scale(rec.n, rec.n);
     int sb = ntt_scale(b.n, 1);
@@ -616,8 +635,18 @@ void dec_big_level(Ctx& c, const DN& cur, u64 sx, u64 j0, u64 count, const DN& b
     launch_shr_pieces(nxt.p + (2 * j0 + 1) * w, 2 * w, lo.p, 2 * w, 2 * w, 0, w, m, c.st);
 }
 
-// Zero-padded digits of x below dec_base[I+1]: 18 * 2^(I+1) characters.
-std::string to_dec(Ctx& c, DN x) {
+// Decimal digits of x (no leading zeros) written to out with a NUL; returns
+// the digit count.  The digits of pieces below dec_base[0] come from the
+// leaf kernel as 18-character groups of the zero-padded 18 * 2^(I+1)-digit
+// representation; only the part from the first nonzero digit is copied.
+u64 to_dec_into(Ctx& c, DN x, char* out, u64 cap) {
+    auto t0 = Clock::now();
+    auto tick = [&](const char* what) {
+        if (!g_trace) return;
+        c.sync();
+        std::fprintf(stderr, "[nnh] to_dec %-28s %9.3f ms\n", what, ms_since(t0));
+        t0 = Clock::now();
+    };
     c.normalize(x);
     DecBases B;
     B.base.push_back(c.word(DEC_BASE0));
@@ -626,7 +655,13 @@ std::string to_dec(Ctx& c, DN x) {
         B.base.push_back(c.sqr(B.base[I + 1]));
         I++;
     }
-    if (I < 0) return std::to_string(c.read(x.p));
+    if (I < 0) {
+        std::string s = std::to_string(c.read(x.p));
+        if (s.size() + 1 > cap) raise(NNH_ERR_ARG, "output buffer too small");
+        std::memcpy(out, s.c_str(), s.size() + 1);
+        return s.size();
+    }
+    tick("bases");
     B.rec.resize(I + 1);
     B.n_shl.assign(I + 1, 0);
     for (int i = DEC_LEAF_LEVEL + 1; i <= I; i++) {  // calc_dec_base (natural.h:909-923)
@@ -634,6 +669,7 @@ std::string to_dec(Ctx& c, DN x) {
         B.n_shl[i] = ns;
         B.rec[i] = reciprocal_plain(c, c.shl(B.base[i], ns), B.base[i].n * 2 + 1);
     }
+    tick("reciprocals");
     const u64 total = 18ull << (I + 1);
     // digits of x: at most floor(bits * log10 2) + 1
     u64 bits = 64 * (x.n - 1) + (64 - std::countl_zero(c.top(x)));
@@ -661,8 +697,10 @@ std::string to_dec(Ctx& c, DN x) {
         if (w <= DEC_WG_MAX_WORDS && rec.n <= w + 16 && sx <= 2 * w) {
             launch_dec_split_wg(cur.p + j0 * sx, sx, count - j0, base.p, w, rec.p, rec.n, B.n_shl[lvl],
                                 nxt.p + 2 * j0 * w, err.p, c.st);
+            tick("level (work-group)");
         } else {
             dec_big_level(c, cur, sx, j0, count, base, rec, B.n_shl[lvl], nxt);
+            tick("level (batched products)");
         }
         cur = nxt;
         sx = w;
@@ -674,12 +712,27 @@ std::string to_dec(Ctx& c, DN x) {
     if (sx > 16) raise(NNH_ERR_DOMAIN, "radix conversion: leaf pieces too long");
     launch_dec_leaves(cur.p + j0 * sx, sx, count - j0, lvl, dd + j0 * pd, c.st);
     if (c.read(err.p)) raise(NNH_ERR_DOMAIN, "radix conversion: quotient estimate out of range");
-    std::string s(total - j0 * pd, '0');
-    hipchk(hipMemcpyAsync(s.data(), dd + j0 * pd, s.size(), hipMemcpyDeviceToHost, c.st), "digits");
+    tick("leaves");
+    // first nonzero digit: scan the leading pieces on the host
+    std::vector<char> head(pd);
+    u64 first = total;
+    for (u64 j = j0; j < count && first == total; j++) {
+        hipchk(hipMemcpyAsync(head.data(), dd + j * pd, pd, hipMemcpyDeviceToHost, c.st), "digits");
+        c.sync();
+        for (u64 k = 0; k < pd; k++)
+            if (head[k] != '0') {
+                first = j * pd + k;
+                break;
+            }
+    }
+    if (first == total) first = total - 1;  // x == 0 is handled above; keep one digit
+    const u64 len = total - first;
+    if (len + 1 > cap) raise(NNH_ERR_ARG, "output buffer too small (" + std::to_string(len + 1) + " bytes needed)");
+    hipchk(hipMemcpyAsync(out, dd + first, len, hipMemcpyDeviceToHost, c.st), "digits");
     c.sync();
-    size_t nz = s.find_first_not_of('0');
-    if (nz == std::string::npos) return "0";
-    return s.substr(nz);
+    out[len] = 0;
+    tick("copy out");
+    return len;
 }
 
 // ---------------------------------------------------------------------------
@@ -700,8 +753,15 @@ void calc_e(Ctx& c, int k, DN& num, DN& den) {
     if (S > E_LEAF_MAX_WORDS) raise(NNH_ERR_ARG, "calc_e: leaf stride too large");
     u64 nb = 1ull << (k - 5);
     DN cn = c.alloc(nb * S), cd = c.alloc(nb * S);
+    auto t0 = Clock::now();
     launch_e_leaves(cn.p, cd.p, nb, S, c.st);
     for (int l = 6; l <= k; l++) {
+        if (g_trace) {
+            c.sync();
+            std::fprintf(stderr, "[nnh] calc_e level %2d (stride %8llu) %9.3f ms\n", l - 1, (unsigned long long)S,
+                         ms_since(t0));
+            t0 = Clock::now();
+        }
         u64 S2 = e_stride(k, l), ncomb = 1ull << (k - l);
         DN nn = c.alloc(ncomb * S2), nd = c.alloc(ncomb * S2);
         if (S <= E_WG_MAX_WORDS && S2 <= 4096) {
@@ -728,8 +788,6 @@ void calc_e(Ctx& c, int k, DN& num, DN& den) {
     den = cd;
 }
 
-using Clock = std::chrono::steady_clock;
-double ms_since(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
 
 template <class F>
 int guarded(F&& f) {
@@ -791,11 +849,7 @@ int nnh_to_dec(char* out, uint64_t cap, uint64_t* len, const uint64_t* a, uint64
     if (!out || !len || !a || !an) return set_error(NNH_ERR_ARG, "null pointer or empty operand");
     return guarded([&](Ctx& c) {
         DN da = c.upload(a, an);
-        std::string s = to_dec(c, da);
-        if (s.size() + 1 > cap) raise(NNH_ERR_ARG, "output buffer too small");
-        std::memcpy(out, s.data(), s.size());
-        out[s.size()] = 0;
-        *len = s.size();
+        *len = to_dec_into(c, da, out, cap);
     });
 }
 
@@ -833,12 +887,8 @@ int nnh_e_digits(char* out, uint64_t cap, uint64_t* len, int k, uint64_t digits,
         c.sync();
         ms[3] = ms_since(t);
         t = Clock::now();
-        std::string s = to_dec(c, q);  // test.cpp:18-19
+        *len = to_dec_into(c, q, out, cap);  // test.cpp:18-19
         ms[4] = ms_since(t);
-        if (s.size() + 1 > cap) raise(NNH_ERR_ARG, "output buffer too small");
-        std::memcpy(out, s.data(), s.size());
-        out[s.size()] = 0;
-        *len = s.size();
         if (stage_ms)
             for (int i = 0; i < NNH_E_STAGES; i++) stage_ms[i] = ms[i];
     });
