@@ -199,6 +199,21 @@ int nnh_pow(uint64_t* dst, uint64_t cap, uint64_t* n, const uint64_t* b, uint64_
  * dec_init/_print_dec/_dec_split, natural.h:904-992); out gets a
  * NUL-terminated string of *len digits. */
 int nnh_to_dec(char* out, uint64_t cap, uint64_t* len, const uint64_t* a, uint64_t an);
+/* n! (factorial, natural.h:795-839): Legendre exponents, for each exponent
+ * bit the product of the odd primes carrying it (a device product tree; the
+ * reference uses a priority queue, same product), ans = ans^2 * P_i, then the
+ * power of two as a shift.  n < 2^32. */
+int nnh_factorial(uint64_t* out, uint64_t cap, uint64_t* n_out, uint64_t n);
+/* calc_pi(num, den, numc, 1, 2^k + 1) (natural.h:874-902), level-synchronous;
+ * sizes[0..2] receive the words of num, den, numc.  k <= 20. */
+int nnh_calc_pi(uint64_t* num, uint64_t* den, uint64_t* numc, uint64_t cap, int k, uint64_t* sizes);
+/* The reference's pi recipe (test.cpp:22-32): calc_pi over [1, 2^k + 1),
+ * sqrt_10005 to acc words (natural.h:859-872), num = 13591409 den - num,
+ * den *= 426880 numc 10^digits, num *= denc, den /= num, decimal output.
+ * stage_ms (NNH_PI_STAGES entries, may be NULL): calc_pi, sqrt, products,
+ * division, decimal output. */
+#define NNH_PI_STAGES 5
+int nnh_pi_digits(char* out, uint64_t cap, uint64_t* len, int k, uint64_t digits, uint64_t acc, double* stage_ms);
 /* calc_e(num, den, 0, 2^k) (natural.h:841-857), level-synchronous. */
 int nnh_calc_e(uint64_t* num, uint64_t num_cap, uint64_t* num_n, uint64_t* den, uint64_t den_cap, uint64_t* den_n,
                int k);

@@ -91,6 +91,9 @@ def lib():
         "nnh_pow": ([_u64p, u64, _u64p, _u64p, u64, u64], i32),
         "nnh_to_dec": ([ctypes.c_char_p, u64, _u64p, _u64p, u64], i32),
         "nnh_calc_e": ([_u64p, u64, _u64p, _u64p, u64, _u64p, i32], i32),
+        "nnh_factorial": ([_u64p, u64, _u64p, u64], i32),
+        "nnh_calc_pi": ([_u64p, _u64p, _u64p, u64, i32, _u64p], i32),
+        "nnh_pi_digits": ([ctypes.c_char_p, u64, _u64p, i32, u64, u64, ctypes.POINTER(ctypes.c_double)], i32),
         "nnh_e_digits": ([ctypes.c_char_p, u64, _u64p, i32, u64, ctypes.POINTER(ctypes.c_double)], i32),
     }
     for name, (args, res) in sigs.items():
@@ -349,6 +352,39 @@ def calc_e(k):
     nn, dn = ctypes.c_uint64(), ctypes.c_uint64()
     _check(lib().nnh_calc_e(_arr(num), cap, ctypes.byref(nn), _arr(den), cap, ctypes.byref(dn), k))
     return num[:nn.value].copy(), den[:dn.value].copy()
+
+
+def factorial(n):
+    """n! (natural.h:795-839), normalised limbs."""
+    import math
+    bits = int(n * max(1.0, math.log2(max(n, 2)))) + 64
+    cap = bits // 64 + 2
+    out = np.zeros(cap, dtype=np.uint64)
+    m = ctypes.c_uint64()
+    _check(lib().nnh_factorial(_arr(out), cap, ctypes.byref(m), n))
+    return out[:m.value].copy()
+
+
+def calc_pi(k):
+    """(num, den, numc) of calc_pi(1, 2^k + 1) (natural.h:874-902)."""
+    cap = (340 << k) // 64 + 64
+    bufs = [np.zeros(cap, dtype=np.uint64) for _ in range(3)]
+    sizes = np.zeros(3, dtype=np.uint64)
+    _check(lib().nnh_calc_pi(_arr(bufs[0]), _arr(bufs[1]), _arr(bufs[2]), cap, k, _arr(sizes)))
+    return tuple(b[:int(n)].copy() for b, n in zip(bufs, sizes))
+
+
+PI_STAGES = ["calc_pi", "sqrt_10005", "products", "div", "to_dec"]
+
+
+def pi_digits(k, digits, acc):
+    """The reference's pi recipe (test.cpp:22-32): (digit string, {stage: ms})."""
+    cap = digits + 64
+    buf = ctypes.create_string_buffer(cap)
+    n = ctypes.c_uint64()
+    ms = (ctypes.c_double * len(PI_STAGES))()
+    _check(lib().nnh_pi_digits(buf, cap, ctypes.byref(n), k, digits, acc, ms))
+    return buf.raw[:n.value].decode(), dict(zip(PI_STAGES, list(ms)))
 
 
 E_STAGES = ["calc_e", "pow", "mul", "div", "to_dec"]

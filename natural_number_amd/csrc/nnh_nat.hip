@@ -373,6 +373,181 @@ void launch_e_combine(const u64* num, const u64* den, u64 S, u64* num2, u64* den
 }
 
 // ---------------------------------------------------------------------------
+// pi by binary splitting (reference calc_pi, natural.h:874-902).
+// Term b: den = b^3 C, numc = (2b-1)(6b-1)(6b-5), num = (A b + B) numc with
+// C = 640320^3 / 24, A = 545140134, B = 13591409.  Pairs combine with
+// num = |numc_l num_r - num_l den_r|, larger ranges with
+// num = num_l den_r + numc_l num_r; den and numc multiply.
+// ---------------------------------------------------------------------------
+typedef unsigned __int128 u128;
+
+// 128 x 128 -> 256-bit product, little-endian words.
+__device__ __forceinline__ void mul_2x2(u64 a0, u64 a1, u64 b0, u64 b1, u64 (&r)[4]) {
+    u64 l00, h00, l01, h01, l10, h10, l11, h11;
+    mul_wide(a0, b0, l00, h00);
+    mul_wide(a0, b1, l01, h01);
+    mul_wide(a1, b0, l10, h10);
+    mul_wide(a1, b1, l11, h11);
+    r[0] = l00;
+    u128 m = (u128)h00 + l01 + l10;
+    r[1] = (u64)m;
+    u128 n = (m >> 64) + (u128)h01 + h10 + l11;
+    r[2] = (u64)n;
+    r[3] = (u64)(n >> 64) + h11;
+}
+
+// b < 2^21: den < 2^117, numc < 2^70, num < 2^121.
+__device__ __forceinline__ void pi_term(u64 b, u128& num, u128& den, u128& numc) {
+    u64 lo, hi;
+    mul_wide(b * b * b, 0x26dd041d878000ull, lo, hi);
+    den = (u128)hi << 64 | lo;
+    mul_wide((2 * b - 1) * (6 * b - 1), 6 * b - 5, lo, hi);
+    numc = (u128)hi << 64 | lo;
+    const u64 f = 0x207e2da6ull * b + 0xcf6371ull;  // < 2^51
+    mul_wide(f, lo, lo, hi);                         // f * numc_lo
+    hi += f * (u64)(numc >> 64);                     // + f * numc_hi * 2^64 (numc_hi < 2^6)
+    num = (u128)hi << 64 | lo;
+}
+
+__global__ __launch_bounds__(256) void k_pi_pairs(u64 first, u64 npairs, u64* num, u64* den, u64* numc, u64 S) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npairs) return;
+    u128 nl, dl, cl, nr, dr, cr;
+    pi_term(first + 2 * j, nl, dl, cl);
+    pi_term(first + 2 * j + 1, nr, dr, cr);
+    u64 A[4], B[4], D[4], C[4];
+    mul_2x2((u64)cl, (u64)(cl >> 64), (u64)nr, (u64)(nr >> 64), A);  // numc_l num_r
+    mul_2x2((u64)nl, (u64)(nl >> 64), (u64)dr, (u64)(dr >> 64), B);  // num_l den_r
+    mul_2x2((u64)dl, (u64)(dl >> 64), (u64)dr, (u64)(dr >> 64), D);
+    mul_2x2((u64)cl, (u64)(cl >> 64), (u64)cr, (u64)(cr >> 64), C);
+    // |A - B|
+    bool a_ge = true;
+    for (int k = 3; k >= 0; k--)
+        if (A[k] != B[k]) {
+            a_ge = A[k] > B[k];
+            break;
+        }
+    u64 X[4], br = 0;
+    for (int k = 0; k < 4; k++) {
+        u64 x = a_ge ? A[k] : B[k], y = a_ge ? B[k] : A[k];
+        u64 d = x - y - br;
+        br = (x < y || (x == y && br)) ? 1 : 0;
+        X[k] = d;
+    }
+    u64* on = num + j * S;
+    u64* od = den + j * S;
+    u64* oc = numc + j * S;
+    for (u64 k = 0; k < S; k++) {
+        on[k] = k < 4 ? X[k] : 0;
+        od[k] = k < 4 ? D[k] : 0;
+        oc[k] = k < 4 ? C[k] : 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pi_combine(const u64* __restrict__ num, const u64* __restrict__ den,
+                                                    const u64* __restrict__ numc, u64 S, u64* num2, u64* den2,
+                                                    u64* numc2, u64 S2, u64 ncomb) {
+    __shared__ u64 nl[E_WG_MAX], dl[E_WG_MAX], cl[E_WG_MAX], nr[E_WG_MAX], dr[E_WG_MAX], cr[E_WG_MAX];
+    __shared__ u64 X[2 * E_WG_MAX];
+    __shared__ u64 c0[2 * E_WG_MAX], c1[2 * E_WG_MAX], c2[2 * E_WG_MAX];
+    __shared__ u64 xs[3][256 + 3];
+    __shared__ u32 wco[2][4];
+    __shared__ u8 fn;
+    const u32 t = threadIdx.x;
+    for (u64 c = blockIdx.x; c < ncomb; c += gridDim.x) {
+        const u64 L = 2 * c * S, R = (2 * c + 1) * S;
+        __syncthreads();
+        for (u64 k = t; k < S; k += 256) {
+            nl[k] = num[L + k];
+            dl[k] = den[L + k];
+            cl[k] = numc[L + k];
+            nr[k] = num[R + k];
+            dr[k] = den[R + k];
+            cr[k] = numc[R + k];
+        }
+        __syncthreads();
+        const u64 nx = 2 * S < S2 ? 2 * S : S2;
+        wg_mul(X, nx, cl, S, nr, S, nullptr, 0, c0, c1, c2, xs, wco, &fn);
+        wg_mul(num2 + c * S2, S2, nl, S, dr, S, X, nx, c0, c1, c2, xs, wco, &fn);
+        wg_mul(den2 + c * S2, S2, dl, S, dr, S, nullptr, 0, c0, c1, c2, xs, wco, &fn);
+        wg_mul(numc2 + c * S2, S2, cl, S, cr, S, nullptr, 0, c0, c1, c2, xs, wco, &fn);
+    }
+}
+
+void launch_pi_pairs(u64 first, u64 npairs, u64* num, u64* den, u64* numc, u64 S, hipStream_t st) {
+    hipLaunchKernelGGL(k_pi_pairs, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, first, npairs, num, den,
+                       numc, S);
+}
+void launch_pi_combine(const u64* num, const u64* den, const u64* numc, u64 S, u64* num2, u64* den2, u64* numc2,
+                       u64 S2, u64 ncomb, hipStream_t st) {
+    hipLaunchKernelGGL(k_pi_combine, dim3(persist_grid(ncomb, 16384)), dim3(256), 0, st, num, den, numc, S, num2,
+                       den2, numc2, S2, ncomb);
+}
+
+// ---------------------------------------------------------------------------
+// Product trees of one-word factors (the prime products of the reference
+// factorial, natural.h:795-839).  Leaves: thread j multiplies factors
+// [16j, 16j+16) (each < 2^32) into TREE_LEAF_WORDS words; levels with
+// operands of at most 256 words run one work-group per product.
+// ---------------------------------------------------------------------------
+constexpr int TREE_LEAF = 16;
+constexpr int TREE_LEAF_WORDS = 9;
+
+__global__ __launch_bounds__(256) void k_tree_leaves(const u64* __restrict__ f, u64 nleaves, u64* out, u64 stride) {
+    const u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nleaves) return;
+    u64 acc[TREE_LEAF_WORDS];
+#pragma unroll
+    for (int k = 0; k < TREE_LEAF_WORDS; k++) acc[k] = k == 0 ? 1 : 0;
+#pragma unroll
+    for (int g = 0; g < TREE_LEAF; g++) {
+        const u64 m = f[j * TREE_LEAF + g];
+        u64 c = 0;
+#pragma unroll
+        for (int k = 0; k < TREE_LEAF_WORDS; k++) {
+            u64 lo, hi;
+            mul_wide(acc[k], m, lo, hi);
+            u64 s = lo + c;
+            hi += s < lo ? 1 : 0;
+            acc[k] = s;
+            c = hi;
+        }
+    }
+    u64* o = out + j * stride;
+#pragma unroll
+    for (int k = 0; k < TREE_LEAF_WORDS; k++)
+        if ((u64)k < stride) o[k] = acc[k];
+    for (u64 k = TREE_LEAF_WORDS; k < stride; k++) o[k] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_tree_combine(const u64* __restrict__ in, u64 S, u64* out, u64 S2,
+                                                      u64 ncomb) {
+    __shared__ u64 l[E_WG_MAX], r[E_WG_MAX];
+    __shared__ u64 c0[2 * E_WG_MAX], c1[2 * E_WG_MAX], c2[2 * E_WG_MAX];
+    __shared__ u64 xs[3][256 + 3];
+    __shared__ u32 wco[2][4];
+    __shared__ u8 fn;
+    const u32 t = threadIdx.x;
+    for (u64 c = blockIdx.x; c < ncomb; c += gridDim.x) {
+        __syncthreads();
+        for (u64 k = t; k < S; k += 256) {
+            l[k] = in[2 * c * S + k];
+            r[k] = in[(2 * c + 1) * S + k];
+        }
+        __syncthreads();
+        wg_mul(out + c * S2, S2, l, S, r, S, nullptr, 0, c0, c1, c2, xs, wco, &fn);
+    }
+}
+
+void launch_tree_leaves(const u64* f, u64 nleaves, u64* out, u64 stride, hipStream_t st) {
+    hipLaunchKernelGGL(k_tree_leaves, dim3((unsigned)((nleaves + 255) / 256)), dim3(256), 0, st, f, nleaves, out,
+                       stride);
+}
+void launch_tree_combine(const u64* in, u64 S, u64* out, u64 S2, u64 ncomb, hipStream_t st) {
+    hipLaunchKernelGGL(k_tree_combine, dim3(persist_grid(ncomb, 16384)), dim3(256), 0, st, in, S, out, S2, ncomb);
+}
+
+// ---------------------------------------------------------------------------
 // Radix conversion.
 // Division of a two-word numerator by 10^18 with a precomputed reciprocal:
 // Moller and Granlund, "Improved division by invariant integers" (IEEE

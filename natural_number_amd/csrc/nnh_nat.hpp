@@ -54,6 +54,21 @@ void launch_e_leaves(u64* num, u64* den, u64 nblocks, u64 stride, hipStream_t st
 void launch_e_combine(const u64* num, const u64* den, u64 S, u64* num2, u64* den2, u64 S2, u64 ncomb,
                       hipStream_t st);
 
+// pi by binary splitting (calc_pi, natural.h:874-902).  Pairs: entry j
+// combines terms first+2j and first+2j+1 (terms < 2^21) into num, den,
+// numc at stride S >= 4.  Levels: entries 2c, 2c+1 (stride S <= 256) into
+// entry c (stride S2 <= 4096), num = num_l den_r + numc_l num_r.
+void launch_pi_pairs(u64 first, u64 npairs, u64* num, u64* den, u64* numc, u64 S, hipStream_t st);
+void launch_pi_combine(const u64* num, const u64* den, const u64* numc, u64 S, u64* num2, u64* den2, u64* numc2,
+                       u64 S2, u64 ncomb, hipStream_t st);
+
+// Product trees of one-word factors (< 2^32): leaves multiply 16 factors
+// each (f holds 16 * nleaves words, padded with 1) into `stride` words
+// (>= 9); a level multiplies entries 2c and 2c+1 (stride S <= 256) into
+// entry c of stride S2 <= 4096.
+void launch_tree_leaves(const u64* f, u64 nleaves, u64* out, u64 stride, hipStream_t st);
+void launch_tree_combine(const u64* in, u64 S, u64* out, u64 S2, u64 ncomb, hipStream_t st);
+
 // Radix conversion (reference _dec_split/_print_dec, natural.h:953-992).
 // Leaves: piece j (< 10^(18*2^(lvl+1)), stride words <= 15) becomes
 // 18*2^(lvl+1) zero-padded digits at out + j * that many chars.

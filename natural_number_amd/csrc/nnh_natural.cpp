@@ -287,6 +287,9 @@ class Ctx {
         return d;
     }
 
+    // natural.h:324-342: the larger minus the smaller (both normalised)
+    DN absdiff(const DN& x, const DN& y) { return cmp(x, y) >= 0 ? sub(x, y) : sub(y, x); }
+
     // ---- shifts (natural.h:264-295) ------------------------------------------
     // Word shifts keep the size structural; bit shifts normalise.  a != 0.
     DN shl(const DN& a, u64 bits) {
@@ -789,6 +792,155 @@ void calc_e(Ctx& c, int k, DN& num, DN& den) {
 }
 
 
+// ---------------------------------------------------------------------------
+// factorial (natural.h:795-839): n! = 2^e2 * prod over bits i (high to low)
+// of [ans = ans^2 * P_i], P_i the product of the odd primes whose exponent
+// in n! has bit i set.  The reference multiplies each P_i through a
+// smallest-first priority queue; the product is the same in any order, so
+// here it is a level-synchronous product tree on the device.
+// ---------------------------------------------------------------------------
+constexpr u64 TREE_LEAF_FACTORS = 16;
+constexpr u64 TREE_LEAF_MAX_WORDS = 9;
+
+DN product_tree(Ctx& c, const std::vector<u64>& f, u32 bits) {
+    if (f.empty()) return c.word(1);
+    if (bits > 32) raise(NNH_ERR_ARG, "product tree: factors must be below 2^32");
+    u64 nleaf = std::bit_ceil((f.size() + TREE_LEAF_FACTORS - 1) / TREE_LEAF_FACTORS);
+    std::vector<u64> padded(nleaf * TREE_LEAF_FACTORS, 1);
+    std::copy(f.begin(), f.end(), padded.begin());
+    DN df = c.upload(padded.data(), padded.size());
+    // entries of level l hold 2^l leaves of 16 factors below 2^bits
+    auto stride = [&](u64 leaves) { return (leaves * TREE_LEAF_FACTORS * bits + 63) / 64 + 1; };
+    u64 S = std::max<u64>(stride(1), TREE_LEAF_MAX_WORDS);
+    DN cur = c.alloc(nleaf * S);
+    launch_tree_leaves(df.p, nleaf, cur.p, S, c.st);
+    for (u64 n = nleaf, leaves = 2; n > 1; n >>= 1, leaves <<= 1) {
+        u64 S2 = stride(leaves), ncomb = n / 2;
+        DN nxt = c.alloc(ncomb * S2);
+        if (S <= E_WG_MAX_WORDS && S2 <= 4096) {
+            launch_tree_combine(cur.p, S, nxt.p, S2, ncomb, c.st);
+        } else {
+            std::vector<IReq> rq(ncomb);
+            for (u64 j = 0; j < ncomb; j++)
+                rq[j] = IReq{nxt.p + j * S2, cur.p + 2 * j * S, S, cur.p + (2 * j + 1) * S, S, S2, nullptr, 0};
+            rcchk(dev_batch(rq.data(), ncomb, c.st));
+        }
+        cur = nxt;
+        S = S2;
+    }
+    cur.n = S;
+    c.normalize(cur);
+    return cur;
+}
+
+DN factorial(Ctx& c, u64 n) {
+    if (n <= 1) return c.word(1);
+    if (n == 2) return c.word(2);
+    if (n >= (1ull << 32)) raise(NNH_ERR_ARG, "factorial: n must be below 2^32");
+    // sieve and Legendre exponents (natural.h:800-816)
+    std::vector<u8> composite(n + 1, 0);
+    std::vector<u64> prime, pexp;
+    for (u64 i = 2; i <= n; i++) {
+        if (!composite[i]) {
+            prime.push_back(i);
+            u64 e = 0;
+            for (u64 m = n; m; m /= i) e += m / i;
+            pexp.push_back(e);
+            for (u64 j = i * i; j <= n; j += i) composite[j] = 1;
+        }
+    }
+    const u32 bits = 64 - std::countl_zero(n);
+    DN ans = c.word(1);
+    for (int i = 63 - std::countl_zero(pexp[1]); i >= 0; i--) {  // natural.h:818-834
+        std::vector<u64> f;
+        for (u64 j = 1; j < prime.size(); j++) {
+            if (pexp[j] < (1ull << i)) break;
+            if (pexp[j] & (1ull << i)) f.push_back(prime[j]);
+        }
+        DN p = product_tree(c, f, bits);
+        ans = c.mul(c.sqr(ans), p);
+    }
+    return c.shl(ans, pexp[0]);  // natural.h:835
+}
+
+// ---------------------------------------------------------------------------
+// pi (natural.h:859-902, recipe test.cpp:22-32).  calc_pi runs
+// level-synchronously over [1, 2^k + 1): one thread per pair of terms
+// (the pair level subtracts), then levels of num = num_l den_r +
+// numc_l num_r, den = den_l den_r, numc = numc_l numc_r at strides from
+// per-level bit bounds.
+// ---------------------------------------------------------------------------
+struct PiLevel {
+    u64 S;
+    DN num, den, numc;
+};
+
+void calc_pi(Ctx& c, int k, DN& num, DN& den, DN& numc) {
+    if (k < 2 || k > 20) raise(NNH_ERR_ARG, "calc_pi: k must be in [2, 20]");
+    auto bl = [](u64 x) { return (double)(64 - std::countl_zero(x)); };
+    const u64 bmax = 1ull << k;
+    double bd = 3 * bl(bmax) + 54, bc = bl(2 * bmax) + 2 * bl(6 * bmax), bn = bl(0x207e2da6ull * bmax + 0xcf6371) + bc;
+    double nd = 2 * bd, nc = 2 * bc, nn = std::max(bc + bn, bn + bd);  // after the pair level
+    auto words = [](double bits) { return (u64)std::ceil(bits / 64.0) + 1; };
+    u64 S = std::max<u64>(words(std::max({nd, nc, nn})), 5);
+    u64 npairs = bmax / 2;
+    DN cn = c.alloc(npairs * S), cd = c.alloc(npairs * S), cc = c.alloc(npairs * S);
+    launch_pi_pairs(1, npairs, cn.p, cd.p, cc.p, S, c.st);
+    for (u64 n = npairs; n > 1; n >>= 1) {
+        double nn2 = std::max(nn + nd, nc + nn) + 1;
+        nd *= 2;
+        nc *= 2;
+        nn = nn2;
+        const u64 S2 = words(std::max({nd, nc, nn})), ncomb = n / 2;
+        DN xn = c.alloc(ncomb * S2), xd = c.alloc(ncomb * S2), xc = c.alloc(ncomb * S2);
+        if (S <= E_WG_MAX_WORDS && S2 <= 4096) {
+            launch_pi_combine(cn.p, cd.p, cc.p, S, xn.p, xd.p, xc.p, S2, ncomb, c.st);
+        } else {
+            // natural.h:893-901: the numc_l num_r product first, then
+            // num_l den_r with it as the CRT addend
+            DN tmp = c.alloc(ncomb * S2);
+            std::vector<IReq> rq(3 * ncomb);
+            for (u64 j = 0; j < ncomb; j++) {
+                const u64 L = 2 * j * S, R = (2 * j + 1) * S;
+                rq[3 * j] = IReq{tmp.p + j * S2, cc.p + L, S, cn.p + R, S, S2, nullptr, 0};
+                rq[3 * j + 1] = IReq{xd.p + j * S2, cd.p + L, S, cd.p + R, S, S2, nullptr, 0};
+                rq[3 * j + 2] = IReq{xc.p + j * S2, cc.p + L, S, cc.p + R, S, S2, nullptr, 0};
+            }
+            rcchk(dev_batch(rq.data(), rq.size(), c.st));
+            rq.resize(ncomb);
+            for (u64 j = 0; j < ncomb; j++) {
+                const u64 L = 2 * j * S, R = (2 * j + 1) * S;
+                rq[j] = IReq{xn.p + j * S2, cn.p + L, S, cd.p + R, S, S2, tmp.p + j * S2, S2};
+            }
+            rcchk(dev_batch(rq.data(), ncomb, c.st));
+        }
+        cn = xn;
+        cd = xd;
+        cc = xc;
+        S = S2;
+    }
+    cn.n = cd.n = cc.n = S;
+    c.normalize(cn);
+    c.normalize(cd);
+    c.normalize(cc);
+    num = cn;
+    den = cd;
+    numc = cc;
+}
+
+// natural.h:859-872
+void sqrt_10005(Ctx& c, u64 acc, DN& num, DN& den) {
+    num = c.word(4001);
+    den = c.word(40);
+    DN k10005 = c.word(10005);
+    while (num.n + den.n - 2 < acc) {
+        DN c1 = c.sqr(den);
+        DN c2 = c.sqr(num);
+        den = c.shl(c.mul(num, den), 1);
+        num = c.add(c.mul(c1, k10005), c2);
+    }
+}
+
 template <class F>
 int guarded(F&& f) {
     int rc = host_init();
@@ -861,6 +1013,57 @@ int nnh_calc_e(uint64_t* num, uint64_t num_cap, uint64_t* num_n, uint64_t* den, 
         calc_e(c, k, dn, dd);
         *num_n = emit(c, dn, num, num_cap);
         *den_n = emit(c, dd, den, den_cap);
+    });
+}
+
+int nnh_factorial(uint64_t* out, uint64_t cap, uint64_t* n_out, uint64_t n) {
+    if (!out || !n_out) return set_error(NNH_ERR_ARG, "null pointer");
+    return guarded([&](Ctx& c) {
+        DN f = factorial(c, n);
+        *n_out = emit(c, f, out, cap);
+    });
+}
+
+int nnh_calc_pi(uint64_t* num, uint64_t* den, uint64_t* numc, uint64_t cap, int k, uint64_t* sizes) {
+    if (!num || !den || !numc || !sizes) return set_error(NNH_ERR_ARG, "null pointer");
+    return guarded([&](Ctx& c) {
+        DN a, b, d;
+        calc_pi(c, k, a, b, d);
+        sizes[0] = emit(c, a, num, cap);
+        sizes[1] = emit(c, b, den, cap);
+        sizes[2] = emit(c, d, numc, cap);
+    });
+}
+
+int nnh_pi_digits(char* out, uint64_t cap, uint64_t* len, int k, uint64_t digits, uint64_t acc, double* stage_ms) {
+    if (!out || !len) return set_error(NNH_ERR_ARG, "null pointer");
+    return guarded([&](Ctx& c) {
+        double ms[NNH_PI_STAGES] = {0, 0, 0, 0, 0};
+        auto t = Clock::now();
+        DN num, den, numc, sn, sd;
+        calc_pi(c, k, num, den, numc);  // test.cpp:24
+        c.sync();
+        ms[0] = ms_since(t);
+        t = Clock::now();
+        sqrt_10005(c, acc, sn, sd);  // test.cpp:25
+        c.sync();
+        ms[1] = ms_since(t);
+        t = Clock::now();
+        num = c.absdiff(c.mul(c.word(0xcf6371), den), num);                              // test.cpp:26
+        den = c.mul(den, c.mul(c.mul(c.word(0x68380), sn), pow(c, c.word(10), digits)));  // test.cpp:27
+        num = c.mul(num, sd);                                                            // test.cpp:28
+        c.sync();
+        ms[2] = ms_since(t);
+        t = Clock::now();
+        DN q, r;
+        divmod(c, den, num, q, r);  // test.cpp:29
+        c.sync();
+        ms[3] = ms_since(t);
+        t = Clock::now();
+        *len = to_dec_into(c, q, out, cap);  // test.cpp:31
+        ms[4] = ms_since(t);
+        if (stage_ms)
+            for (int i = 0; i < NNH_PI_STAGES; i++) stage_ms[i] = ms[i];
     });
 }
 

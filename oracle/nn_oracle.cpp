@@ -1074,6 +1074,53 @@ void calc_e(Nat& num, Nat& den, u64 begin, u64 end) {
     num = add(num, num1);
 }
 
+// natural.h:859-872
+void sqrt_10005(Nat& num, Nat& den, u64 acc) {
+    num = Nat(4001);
+    den = Nat(40);
+    Nat cache1, cache2;
+    while (num.size + den.size - 2 < acc) {
+        NttInfo ni_num, ni_den;
+        ni_num.scale = ni_den.scale = ntt_scale(num.size, num.size);
+        sqr_save(cache1, den, ni_den);
+        sqr_save(cache2, num, ni_num);
+        mul_load2(den, num, den, ni_num, ni_den);
+        den = shl(den, 1);
+        num = add(mul(cache1, Nat(10005)), cache2);
+    }
+}
+
+// natural.h:874-902
+void calc_pi(Nat& num, Nat& den, Nat& numc, u64 begin, u64 end) {
+    if (end - begin == 1) {
+        den = Nat(begin);
+        den = mul(mul(mul(den, den), den), Nat(0x26dd041d878000ull));
+        numc = Nat(2 * begin - 1);
+        numc = mul(numc, Nat(6 * begin - 1));
+        numc = mul(numc, Nat(6 * begin - 5));
+        num = Nat(begin);
+        num = mul(add(mul(Nat(0x207e2da6ull), num), Nat(0xcf6371)), numc);
+        return;
+    }
+    Nat num1, den1, numc1;
+    calc_pi(num, den, numc, begin, (begin + end) >> 1);
+    calc_pi(num1, den1, numc1, (begin + end) >> 1, end);
+    if (end - begin == 2) {
+        num = sub(mul(numc, num1), mul(num, den1));  // natural subtraction: |a - b|
+        den = mul(den, den1);
+        numc = mul(numc, numc1);
+        return;
+    }
+    NttInfo ni_den1, ni_numc;
+    ni_den1.scale = ntt_scale(std::max(den.size, num.size), den1.size);
+    mul_save(num, den1, num, ni_den1);
+    mul_load(den, den1, den, ni_den1);
+    ni_numc.scale = ntt_scale(std::max(numc1.size, num1.size), numc.size);
+    mul_save(num1, numc, num1, ni_numc);
+    mul_load(numc, numc, numc1, ni_numc);
+    num = add(num, num1);
+}
+
 // ---- decimal output: natural.h:904-1039 ------------------------------------
 static const u64 DEC_LEN = 18, DEC_BASE0 = 1000000000000000000ull;  // natural.h:904
 static Nat dec_base[64], dec_base_r[64];                             // natural.h:905
@@ -1295,6 +1342,35 @@ u64 nno_e_digits(char* out, u64 cap, int k, u64 digits) {
     Nat q, r;
     divmod(num, den, q, r);
     return nno_to_dec(out, cap, q.data, q.size);
+}
+// The reference's pi recipe (test.cpp:22-32): calc_pi over [1, 2^k + 1),
+// sqrt(10005) to acc words, num = 13591409 den - num, den *= 426880 numc
+// 10^digits, num *= denc, den /= num, decimal digits of den.
+u64 nno_pi_digits(char* out, u64 cap, int k, u64 digits, u64 acc) {
+    Nat num, den, numc, denc;
+    calc_pi(num, den, numc, 1, (1ull << k) | 1);
+    sqrt_10005(numc, denc, acc);
+    num = sub(mul(Nat(0xcf6371), den), num);
+    den = mul(den, mul(mul(Nat(0x68380), numc), pow(Nat(10), digits)));
+    num = mul(num, denc);
+    Nat q, r;
+    divmod(den, num, q, r);
+    return nno_to_dec(out, cap, q.data, q.size);
+}
+// calc_pi over [1, 2^k + 1): num, den, numc (natural.h:874-902).
+void nno_calc_pi(u64* num, u64* den, u64* numc, u64 cap, int k, u64* sizes) {
+    Nat a, b, c;
+    calc_pi(a, b, c, 1, (1ull << k) | 1);
+    sizes[0] = emit(a, num, cap);
+    sizes[1] = emit(b, den, cap);
+    sizes[2] = emit(c, numc, cap);
+}
+// sqrt_10005(num, den, acc) (natural.h:859-872).
+void nno_sqrt_10005(u64* num, u64* den, u64 cap, u64 acc, u64* sizes) {
+    Nat a, b;
+    sqrt_10005(a, b, acc);
+    sizes[0] = emit(a, num, cap);
+    sizes[1] = emit(b, den, cap);
 }
 // calc_e numerator/denominator for [0, 2^k); sizes returned through pointers.
 void nno_calc_e(u64* num, u64 num_cap, u64* den, u64 den_cap, int k, u64* num_n, u64* den_n) {

@@ -170,3 +170,49 @@ def test_e_digits_vs_oracle(nnh, oracle):
     s, _ = nnh.e_digits(12, 3000)
     assert s == oracle.e_digits(12, 3000)
     assert s.startswith("271828182845904523536028747135266249775724709369995")
+
+
+# ---- factorial (natural.h:795-839) ---------------------------------------------
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 4, 10, 31, 100, 1000, 20000, 200000])
+def test_factorial_exact(nnh, n):
+    import math
+    assert to_int(nnh.factorial(n)) == math.factorial(n)
+
+
+@pytest.mark.parametrize("n", ["1000000", "10000000"])
+def test_factorial_golden(nnh, n):
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "factorial.json")))[n]
+    f = nnh.factorial(int(n))
+    assert f.size == g["words"] and hex(int(f[-1])) == g["top"]
+    assert hashlib.sha256(f.astype("<u8").tobytes()).hexdigest() == g["sha256"]
+
+
+# ---- pi (natural.h:859-902, test.cpp:22-32) --------------------------------------
+@pytest.mark.parametrize("k", [2, 4, 8, 12, 16])
+def test_calc_pi_matches_oracle(nnh, oracle, k):
+    import ctypes
+    num, den, numc = nnh.calc_pi(k)
+    cap = (340 << k) // 64 + 64
+    bufs = [np.zeros(cap, dtype=np.uint64) for _ in range(3)]
+    sizes = np.zeros(3, dtype=np.uint64)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    oracle.L.nno_calc_pi.argtypes = [u64p, u64p, u64p, ctypes.c_uint64, ctypes.c_int, u64p]
+    oracle.L.nno_calc_pi(*(b.ctypes.data_as(u64p) for b in bufs), cap, k, sizes.ctypes.data_as(u64p))
+    for got, b, n in zip((num, den, numc), bufs, sizes):
+        assert np.array_equal(got, b[:int(n)])
+
+
+def test_pi_digits_small_exact(nnh):
+    from test_oracle import _py_pi_digits
+    for k, D, acc in [(4, 100, 8), (6, 500, 30), (8, 2000, 120)]:
+        s, _ = nnh.pi_digits(k, D, acc)
+        assert s == _py_pi_digits(k, D, acc)
+
+
+@pytest.mark.parametrize("name", ["pi_k14_D200000_acc12000", "pi_k17_D1000000_acc60000",
+                                  "pi_k20_D10000000_acc600000"])
+def test_pi_digits_golden(nnh, name):
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "pi_digits.json")))[name]
+    s, _ = nnh.pi_digits(g["k"], g["digits"], g["acc"])
+    assert len(s) == g["length"] and s[:60] == g["head"] and s[-60:] == g["tail"]
+    assert hashlib.sha256(s.encode()).hexdigest() == g["sha256"]

@@ -2,6 +2,7 @@
 reference CPU path — by checks that do not reuse its formulas: the
 reference's own literals, closed forms, exact Python integers and brute-force
 polynomial evaluation on tiny transforms."""
+import ctypes
 import random
 
 import numpy as np
@@ -299,3 +300,60 @@ def test_e_digits_exact(oracle):
         num += t
     assert got == str(num * 10**D // den)
     assert got.startswith("27182818284590452353602874713526624977572470936999595749669676277240766303535")
+
+
+# ---- pi (natural.h:859-902, test.cpp:22-32) -------------------------------------
+def _words(x):
+    return max(1, (x.bit_length() + 63) // 64)
+
+
+def _py_calc_pi(b, e):
+    """Exact-integer restatement of calc_pi (natural.h:874-902)."""
+    if e - b == 1:
+        den = b * b * b * 0x26DD041D878000
+        numc = (2 * b - 1) * (6 * b - 1) * (6 * b - 5)
+        num = (0x207E2DA6 * b + 0xCF6371) * numc
+        return num, den, numc
+    m = (b + e) >> 1
+    num, den, numc = _py_calc_pi(b, m)
+    num1, den1, numc1 = _py_calc_pi(m, e)
+    if e - b == 2:
+        return abs(numc * num1 - num * den1), den * den1, numc * numc1
+    return num * den1 + numc * num1, den * den1, numc * numc1
+
+
+def _py_pi_digits(k, digits, acc):
+    num, den, numc = _py_calc_pi(1, (1 << k) | 1)
+    sn, sd = 4001, 40                       # sqrt_10005 (natural.h:859-872)
+    while _words(sn) + _words(sd) - 2 < acc:
+        sn, sd = sd * sd * 10005 + sn * sn, (sn * sd) << 1
+    num = abs(0xCF6371 * den - num)
+    den = den * (0x68380 * sn * 10**digits)
+    num = num * sd
+    return str(den // num)
+
+
+def _oracle_pi(oracle, k, digits, acc):
+    buf = ctypes.create_string_buffer(digits + 64)
+    oracle.L.nno_pi_digits.restype = ctypes.c_uint64
+    oracle.L.nno_pi_digits.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                                       ctypes.c_uint64]
+    n = oracle.L.nno_pi_digits(buf, digits + 64, k, digits, acc)
+    return buf.raw[:n].decode()
+
+
+@pytest.mark.parametrize("k,digits,acc", [(4, 100, 8), (6, 500, 30), (8, 2000, 120), (10, 6000, 350)])
+def test_pi_recipe_exact(oracle, k, digits, acc):
+    assert _oracle_pi(oracle, k, digits, acc) == _py_pi_digits(k, digits, acc)
+
+
+def test_pi_digits_vs_mpmath(oracle):
+    """The recipe's digits are pi's (up to the last few, which the truncated
+    series and square root may not fix): independent pin by mpmath."""
+    import mpmath
+    D = 20000
+    got = _oracle_pi(oracle, 11, D, 1200)
+    mpmath.mp.dps = D + 30
+    ref = mpmath.nstr(mpmath.pi, D + 20, strip_zeros=False).replace(".", "")
+    assert got.startswith("31415926535897932384626433832795028841971693993751")
+    assert got[:D - 5] == ref[:D - 5]
