@@ -146,6 +146,52 @@ def e_workload(nn, with_cpu):
     return res
 
 
+def drivers_workload(nn, with_cpu):
+    """The reference's other binary-splitting drivers: the pi recipe
+    (test.cpp:22-32) at 10^7 digits and factorial(10^7) (test.cpp:35-39),
+    GPU wall times, checked against the committed digests; the oracle's pi
+    is timed at 10^6 digits."""
+    with open(os.path.join(ROOT, "tests", "golden", "pi_digits.json")) as f:
+        gp = json.load(f)
+    with open(os.path.join(ROOT, "tests", "golden", "factorial.json")) as f:
+        gf = json.load(f)
+    res = {}
+    nn.pi_digits(14, 200000, 12000)  # warm-up
+    for name in ("pi_k17_D1000000_acc60000", "pi_k20_D10000000_acc600000"):
+        g = gp[name]
+        log(f"pi: {g['digits']} digits on the GPU")
+        t = time.perf_counter()
+        s, stages = nn.pi_digits(g["k"], g["digits"], g["acc"])
+        wall = time.perf_counter() - t
+        res[name] = {"gpu_wall_s": round(wall, 4), "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+                     "digits_match_reference": hashlib.sha256(s.encode()).hexdigest() == g["sha256"]}
+        del s
+    if with_cpu:
+        g = gp["pi_k17_D1000000_acc60000"]
+        log("pi: oracle on 10^6 digits (1 thread)")
+        lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+        lib.nno_pi_digits.restype = ctypes.c_uint64
+        lib.nno_pi_digits.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                                      ctypes.c_uint64]
+        buf = ctypes.create_string_buffer(g["digits"] + 64)
+        t = time.perf_counter()
+        n = lib.nno_pi_digits(buf, g["digits"] + 64, g["k"], g["digits"], g["acc"])
+        tcpu = time.perf_counter() - t
+        res["cpu_oracle_pi_k17_D1000000"] = {"seconds": round(tcpu, 3), "cores": 1,
+                                             "digits_match_reference": hashlib.sha256(buf.raw[:n]).hexdigest()
+                                             == g["sha256"]}
+        res["pi_speedup_vs_cpu_at_1e6_digits"] = round(tcpu / res["pi_k17_D1000000_acc60000"]["gpu_wall_s"], 1)
+    n = 10**7
+    log("factorial(10^7) on the GPU")
+    t = time.perf_counter()
+    f = nn.factorial(n)
+    wall = time.perf_counter() - t
+    res["factorial_1e7"] = {"gpu_wall_s": round(wall, 4), "words": int(f.size),
+                            "matches_gmp": hashlib.sha256(f.astype("<u8").tobytes()).hexdigest()
+                            == gf[str(n)]["sha256"]}
+    return res
+
+
 def cost_model_run(n_side, n_held):
     """Config 5: timings over a log-uniform (m, n) grid of 2^10 .. 2^30-bit
     operands and batch counts, the FP32 MLP trained on the GPU by the
@@ -206,6 +252,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-mix measurement")
     ap.add_argument("--no-e", action="store_true", help="skip the e-digits workload")
+    ap.add_argument("--no-drivers", action="store_true", help="skip the pi and factorial workloads")
     ap.add_argument("--cost-model", action="store_true",
                     help="only run the cost-model pipeline (grid timings, on-device training, held-out regret)")
     ap.add_argument("--held-out", type=int, default=10000, help="held-out requests for --cost-model")
@@ -296,6 +343,7 @@ def main():
 
     batch = None if args.no_batch else batch_mix(nn, stream)
     e_res = None if args.no_e else e_workload(nn, not args.no_cpu)
+    drv = None if args.no_drivers else drivers_workload(nn, not args.no_cpu)
 
     res = out.download()
     gpu_digest = hashlib.sha256(res.tobytes()).hexdigest()
@@ -354,6 +402,7 @@ def main():
         "cpu_baseline": cpu,
         "batch_mix": batch,
         "e_digits": e_res,
+        "binary_splitting_drivers": drv,
         "bit_exact_vs_cpu": exact,
         "result_sha256": gpu_digest,
         "wall_s_timed": wall,

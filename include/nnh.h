@@ -110,8 +110,8 @@ int nnh_mul_cyclic(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t
 
 /* Forward transform of a at `scale` (NTT_info::load + NTT_info::NTT,
  * ntt_info.h:147-198) into out[3 * 2^scale] (prime-major).  The words are
- * the reference's transform multiplied by alpha_scale (see nnh_alpha) and
- * may be lazily reduced into [0, 2p).  Host buffers. */
+ * the reference's transform multiplied by alpha_scale (see nnh_alpha),
+ * lazily reduced: in [0, 4p) for p0 and p1, [0, 2p) for p2.  Host buffers. */
 int nnh_transform_forward(uint64_t* out, const uint64_t* a, uint64_t an, int scale);
 
 /* The constant alpha_scale mod p_i (i = 0..2) by which device transforms

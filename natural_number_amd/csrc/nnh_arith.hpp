@@ -35,10 +35,14 @@ __host__ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi
 // One NTT prime p = k*2^56 + 1 (held in scalar registers inside kernels).
 struct Prime {
     u64 p;
+    u64 p2;   // 2p
     u32 k;
     u32 k24;  // k << 24
 };
-__host__ __device__ constexpr Prime make_prime(u32 k) { return Prime{prime_p(k), k, k << 24}; }
+__host__ __device__ constexpr Prime make_prime(u32 k) { return Prime{prime_p(k), prime_p(k) << 1, k, k << 24}; }
+// 4p < 2^64 holds for p0 and p1 (k = 27, 58) but not p2 (k = 87): only the
+// first two primes use the lazy [0, 4p) butterflies below.
+__host__ __device__ constexpr bool lazy_prime(int i) { return i < 2; }
 __host__ __device__ constexpr Prime prime_of(int i) { return make_prime(PRIME_K[i]); }
 
 // Montgomery reduction for p = k*2^56+1, R = 2^64: returns T * 2^-64 mod p
@@ -79,6 +83,31 @@ __host__ __device__ __forceinline__ u64 csub(const Prime& P, u64 x) {
     return (i64)y < 0 ? x : y;
 }
 
+// x in [0, 4p) -> [0, 2p) (requires 2p < 2^63)
+__host__ __device__ __forceinline__ u64 csub2(const Prime& P, u64 x) {
+    u64 y = x - P.p2;
+    return (i64)y < 0 ? x : y;
+}
+
+// Lazy butterflies after Harvey ("Faster arithmetic for number-theoretic
+// transforms", J. Symbolic Comput. 60, 2014) for primes with 4p < 2^64:
+// Montgomery products of inputs below 4p stay below 2p, so one conditional
+// subtraction of 2p per butterfly replaces two of p.
+// Forward: (x, y) in [0, 4p) -> (x + w*y, x - w*y) in [0, 4p).
+__host__ __device__ __forceinline__ void ct_bfly_lazy(const Prime& P, u64& x, u64& y, u64 wm) {
+    u64 t = mont(P, y, wm);  // y*w < 4p*p < p*2^64: t in [0, 2p)
+    u64 a = csub2(P, x);
+    x = a + t;
+    y = a + P.p2 - t;
+}
+// Inverse: (x, y) in [0, 2p) -> (x + y, (x - y) * w) in [0, 2p).
+__host__ __device__ __forceinline__ void gs_bfly_lazy(const Prime& P, u64& x, u64& y, u64 wm) {
+    u64 a = csub2(P, x + y);
+    u64 d = x + P.p2 - y;  // (0, 4p)
+    y = mont(P, d, wm);
+    x = a;
+}
+
 // Cooley-Tukey butterfly (x, y) -> (x + w*y, x - w*y), inputs and outputs in
 // [0, 2p); wm = w * 2^64 mod p.
 __host__ __device__ __forceinline__ void ct_bfly(const Prime& P, u64& x, u64& y, u64 wm) {
@@ -97,10 +126,16 @@ __host__ __device__ __forceinline__ void gs_bfly(const Prime& P, u64& x, u64& y,
 
 // Base multiply of one 4-word group (reference asmNttMul0/1/2,
 // asmLinux.asm:910-1367): (x0+x1X)(y0+y1X) mod (X^2 - w) and
-// (x2+x3X)(y2+y3X) mod (X^2 + w).  Inputs in [0, 2p); outputs canonical and
-// carrying a factor 2^-64 (absorbed by the load scaling).
+// (x2+x3X)(y2+y3X) mod (X^2 + w).  Inputs in [0, 2p), or [0, 4p) when LAZY;
+// outputs canonical and carrying a factor 2^-64 (absorbed by the load
+// scaling).
+template <bool LAZY = false>
 __host__ __device__ __forceinline__ void base_mul4(const Prime& P, u64& x0, u64& x1, u64& x2, u64& x3, u64 y0,
                                                    u64 y1, u64 y2, u64 y3, u64 wm) {
+    if constexpr (LAZY) {
+        x0 = csub2(P, x0); x1 = csub2(P, x1); x2 = csub2(P, x2); x3 = csub2(P, x3);
+        y0 = csub2(P, y0); y1 = csub2(P, y1); y2 = csub2(P, y2); y3 = csub2(P, y3);
+    }
     x0 = csub(P, x0); x1 = csub(P, x1); x2 = csub(P, x2); x3 = csub(P, x3);
     y0 = csub(P, y0); y1 = csub(P, y1); y2 = csub(P, y2); y3 = csub(P, y3);
     u64 t1 = csub(P, mont(P, y1, wm));        // w*y1

@@ -36,7 +36,12 @@ constexpr int CRT_TILE_ROUNDS = 16;  // 16 x 256 positions per tile
 __device__ __forceinline__ void crt_x(const CrtArgs& A, u64 k, u64& x0, u64& x1, u64& x2) {
     if (k >= A.n_res) x0 = x1 = x2 = 0;
     else if (A.columns) { x0 = A.r[0][k]; x1 = A.r[1][k]; x2 = A.r[2][k]; }
-    else garner(A.r[0][k], A.r[1][k], A.r[2][k], A.C.c02m, A.C.m0m, A.C.c1m, A.C.m02lo, A.C.m02hi, x0, x1, x2);
+    else {
+        // p0 and p1 leave the inverse transform in [0, 2p) (lazy butterflies)
+        constexpr Prime Q0 = prime_of(0), Q1 = prime_of(1);
+        garner(csub(Q0, A.r[0][k]), csub(Q1, A.r[1][k]), A.r[2][k], A.C.c02m, A.C.m0m, A.C.c1m, A.C.m02lo,
+               A.C.m02hi, x0, x1, x2);
+    }
 }
 
 __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn_slot, u64 (*xs)[256 + 3],

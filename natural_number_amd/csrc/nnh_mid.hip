@@ -1,101 +1,21 @@
-// nnh_mid.hip — the fused chunk kernel (remaining forward levels of both
-// operands, base multiply, inverse levels h < 4096), single and batched.
-#include "nnh_tile.hpp"
+// nnh_mid.hip — fused chunk kernels for p2 (strict butterflies) and the
+// dispatch of one chunk pass over all three primes.
+#include "nnh_mid_kernels.hpp"
 
 namespace nnh {
 
-template <int CB, bool FROM_LIMBS, bool SQR, bool FWD_ONLY>
-__global__ __launch_bounds__(1 << (CB - 4)) void k_mid(MidArgs A) {
-    __shared__ u64 lds[(1 << CB) + (1 << (CB - 4))];
-    const u32 pr = blockIdx.y;
-    mid_body<CB, FROM_LIMBS, SQR, FWD_ONLY, false>(A, pr, blockIdx.x, lds);
-}
-
-
-template <int CB, bool FROM_LIMBS>
-__global__ __launch_bounds__(1 << (CB - 4)) void k_mid_batch(const BJob* __restrict__ jobs, u32 njobs,
-                                                            MidBatchParams P) {
-    __shared__ u64 lds[(1 << CB) + (1 << (CB - 4))];
-    const u64 cpj = P.L >> CB;
-    const u64 items = (u64)njobs * cpj * 3;
-    for (u64 it = blockIdx.x; it < items; it += gridDim.x) {
-        const u32 pr = (u32)(it % 3);
-        const u64 r = it / 3;
-        const u64 chunk = r % cpj;
-        const BJob& J = jobs[r / cpj];
-        MidArgs A;
-        for (int i = 0; i < 3; i++) {
-            A.fa[i] = J.fa + (u64)i * P.L;
-            A.fb[i] = J.fb + (u64)i * P.L;
-            A.twf[i] = P.twf[i];
-            A.twi[i] = P.twi[i];
-            A.alpha_m[i] = P.alpha_m[i];
-        }
-        A.la = J.a;
-        A.lb = FROM_LIMBS ? J.b : nullptr;
-        A.an = J.an;
-        A.bn = J.bn;
-        A.a_done = 0;
-        A.b_done = 0;
-        bool sq = J.sqr != 0;
-        mid_body<CB, FROM_LIMBS, false, false, true>(A, pr, chunk, lds, sq);
-        __syncthreads();
-    }
-}
-
-
-template <int CB>
-static void mid_dispatch(const MidArgs& A, u64 L, bool from_limbs, bool sqr, bool fwd_only, hipStream_t st) {
-    dim3 grid((unsigned)(L >> CB), 3), block(1 << (CB - 4));
-    if (fwd_only) {
-        if (from_limbs) hipLaunchKernelGGL((k_mid<CB, true, false, true>), grid, block, 0, st, A);
-        else hipLaunchKernelGGL((k_mid<CB, false, false, true>), grid, block, 0, st, A);
-    } else if (from_limbs) {
-        if (sqr) hipLaunchKernelGGL((k_mid<CB, true, true, false>), grid, block, 0, st, A);
-        else hipLaunchKernelGGL((k_mid<CB, true, false, false>), grid, block, 0, st, A);
-    } else {
-        if (sqr) hipLaunchKernelGGL((k_mid<CB, false, true, false>), grid, block, 0, st, A);
-        else hipLaunchKernelGGL((k_mid<CB, false, false, false>), grid, block, 0, st, A);
-    }
-}
+void launch_mid_part(const MidDesc& d, bool lazy, hipStream_t st);
+void launch_mid_batch_part(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs, bool lazy,
+                           hipStream_t st);
 
 void launch_mid(const MidDesc& D, hipStream_t st) {
-    MidArgs A;
-    for (int i = 0; i < 3; i++) {
-        A.fa[i] = D.fa[i];
-        A.fb[i] = D.fb[i];
-        A.twf[i] = D.twf[i];
-        A.twi[i] = D.twi[i];
-        A.alpha_m[i] = D.alpha_m[i];
-    }
-    A.la = D.la; A.lb = D.lb; A.an = D.an; A.bn = D.bn;
-    A.a_done = D.a_done; A.b_done = D.b_done;
-    bool fl = D.la != nullptr;
-    switch (D.cb) {
-        case 8: mid_dispatch<8>(A, D.L, fl, D.sqr, D.fwd_only, st); break;
-        case 9: mid_dispatch<9>(A, D.L, fl, D.sqr, D.fwd_only, st); break;
-        case 10: mid_dispatch<10>(A, D.L, fl, D.sqr, D.fwd_only, st); break;
-        case 11: mid_dispatch<11>(A, D.L, fl, D.sqr, D.fwd_only, st); break;
-        default: mid_dispatch<12>(A, D.L, fl, D.sqr, D.fwd_only, st); break;
-    }
-}
-
-template <int CB>
-static void mid_batch_dispatch(const BJob* jobs, u32 njobs, const MidBatchParams& P, bool from_limbs, hipStream_t st) {
-    u64 items = (u64)njobs * (P.L >> CB) * 3;
-    dim3 grid(persist_grid(items, 4096)), block(1 << (CB - 4));
-    if (from_limbs) hipLaunchKernelGGL((k_mid_batch<CB, true>), grid, block, 0, st, jobs, njobs, P);
-    else hipLaunchKernelGGL((k_mid_batch<CB, false>), grid, block, 0, st, jobs, njobs, P);
+    launch_mid_part(D, true, st);
+    mid_primes<false>(D, st);
 }
 
 void launch_mid_batch(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs, hipStream_t st) {
-    switch (cb) {
-        case 8: mid_batch_dispatch<8>(jobs, njobs, P, from_limbs, st); break;
-        case 9: mid_batch_dispatch<9>(jobs, njobs, P, from_limbs, st); break;
-        case 10: mid_batch_dispatch<10>(jobs, njobs, P, from_limbs, st); break;
-        case 11: mid_batch_dispatch<11>(jobs, njobs, P, from_limbs, st); break;
-        default: mid_batch_dispatch<12>(jobs, njobs, P, from_limbs, st); break;
-    }
+    launch_mid_batch_part(jobs, njobs, P, cb, from_limbs, true, st);
+    mid_batch_primes<false>(jobs, njobs, P, cb, from_limbs, st);
 }
 
 }  // namespace nnh

@@ -61,7 +61,7 @@ __device__ __forceinline__ void xchg(u64 (&v)[16], u64* lds, u32 t) {
 // One level at local bit L with the register window at LO (L-LO in 0..3).
 // Block index of element y: b = tile_id 2^(CB-L-1) + (y >> (L+1)); for the
 // thread's elements this is a per-thread base plus j >> (L+1-LO).
-template <bool INV, int CB, int L, int LO>
+template <bool INV, int CB, int L, int LO, bool LAZY>
 __device__ __forceinline__ void level_ct(const Prime& P, u64 (&v)[16], u32 t, u64 tile_id, const u64* __restrict__ tw) {
     static_assert(L >= LO && L <= LO + 3, "level outside the register window");
     constexpr int D = 1 << (L - LO);
@@ -70,7 +70,9 @@ __device__ __forceinline__ void level_ct(const Prime& P, u64 (&v)[16], u32 t, u6
         constexpr int j = decltype(jc)::value;
         if constexpr ((j & D) == 0) {
             const u64 w = twb[j >> (L + 1 - LO)];
-            if constexpr (INV) gs_bfly(P, v[j], v[j + D], w);
+            if constexpr (INV && LAZY) gs_bfly_lazy(P, v[j], v[j + D], w);
+            else if constexpr (INV) gs_bfly(P, v[j], v[j + D], w);
+            else if constexpr (LAZY) ct_bfly_lazy(P, v[j], v[j + D], w);
             else ct_bfly(P, v[j], v[j + D], w);
         }
     });
@@ -92,24 +94,24 @@ __host__ __device__ constexpr int inv_window(int l, int l_lo, int lo0, int cb) {
     return lo;
 }
 
-template <int CB, int L, int L_LO, int L_HI, int LO0, int LO_MIN>
+template <int CB, int L, int L_LO, int L_HI, int LO0, int LO_MIN, bool LAZY>
 __device__ __forceinline__ void fwd_rec(const Prime& P, u64 (&v)[16], u64* lds, u32 t, u64 tile_id, const u64* __restrict__ tw) {
     if constexpr (L >= L_LO) {
         constexpr int prev = L == L_HI ? LO0 : fwd_window(L + 1, L_HI, LO0, LO_MIN);
         constexpr int cur = fwd_window(L, L_HI, LO0, LO_MIN);
         if constexpr (cur != prev) xchg<CB, prev, cur>(v, lds, t);
-        level_ct<false, CB, L, cur>(P, v, t, tile_id, tw);
-        fwd_rec<CB, L - 1, L_LO, L_HI, LO0, LO_MIN>(P, v, lds, t, tile_id, tw);
+        level_ct<false, CB, L, cur, LAZY>(P, v, t, tile_id, tw);
+        fwd_rec<CB, L - 1, L_LO, L_HI, LO0, LO_MIN, LAZY>(P, v, lds, t, tile_id, tw);
     }
 }
-template <int CB, int L, int L_LO, int L_HI, int LO0>
+template <int CB, int L, int L_LO, int L_HI, int LO0, bool LAZY>
 __device__ __forceinline__ void inv_rec(const Prime& P, u64 (&v)[16], u64* lds, u32 t, u64 tile_id, const u64* __restrict__ tw) {
     if constexpr (L <= L_HI) {
         constexpr int prev = L == L_LO ? LO0 : inv_window(L - 1, L_LO, LO0, CB);
         constexpr int cur = inv_window(L, L_LO, LO0, CB);
         if constexpr (cur != prev) xchg<CB, prev, cur>(v, lds, t);
-        level_ct<true, CB, L, cur>(P, v, t, tile_id, tw);
-        inv_rec<CB, L + 1, L_LO, L_HI, LO0>(P, v, lds, t, tile_id, tw);
+        level_ct<true, CB, L, cur, LAZY>(P, v, t, tile_id, tw);
+        inv_rec<CB, L + 1, L_LO, L_HI, LO0, LAZY>(P, v, lds, t, tile_id, tw);
     }
 }
 
@@ -142,7 +144,7 @@ struct TopArgs {
 
 // Top pass over a 4096-element tile of Q = 2^(12-NLEV) consecutive columns
 // by R = 2^NLEV rows of stride d = 2^g_lo (local index y = col + Q row).
-template <bool INV, bool FROM_LIMBS, int NLEV>
+template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY>
 __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile, u64* lds) {
     const Prime P = make_prime(PRIME_K[0] * (pr == 0) + PRIME_K[1] * (pr == 1) + PRIME_K[2] * (pr == 2));
     constexpr int CB = 12, Q = 12 - NLEV;
@@ -172,10 +174,10 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
         });
     }
     if constexpr (!INV) {
-        fwd_rec<CB, CB - 1, Q, CB - 1, LO_IO, Q>(P, v, lds, t, blk, tw);
+        fwd_rec<CB, CB - 1, Q, CB - 1, LO_IO, Q, LAZY>(P, v, lds, t, blk, tw);
     } else {
         if constexpr (LO_INV0 != LO_IO) xchg<CB, LO_IO, LO_INV0>(v, lds, t);
-        inv_rec<CB, Q, Q, CB - 1, LO_INV0>(P, v, lds, t, blk, tw);
+        inv_rec<CB, Q, Q, CB - 1, LO_INV0, LAZY>(P, v, lds, t, blk, tw);
     }
     if constexpr (LO_ST != LO_END) xchg<CB, LO_END, LO_ST>(v, lds, t);
     {
@@ -186,20 +188,6 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
             const u64 x = xbase + (y & QM) + (u64)(y >> Q) * d;
             data[x] = v[decltype(jc)::value];
         });
-    }
-}
-
-template <bool INV, bool FROM_LIMBS>
-__device__ __forceinline__ void top_pass_nlev(const TopArgs& A, u32 pr, u64 tile, u64* lds) {
-    switch (A.nlev) {
-        case 1: top_pass_body<INV, FROM_LIMBS, 1>(A, pr, tile, lds); break;
-        case 2: top_pass_body<INV, FROM_LIMBS, 2>(A, pr, tile, lds); break;
-        case 3: top_pass_body<INV, FROM_LIMBS, 3>(A, pr, tile, lds); break;
-        case 4: top_pass_body<INV, FROM_LIMBS, 4>(A, pr, tile, lds); break;
-        case 5: top_pass_body<INV, FROM_LIMBS, 5>(A, pr, tile, lds); break;
-        case 6: top_pass_body<INV, FROM_LIMBS, 6>(A, pr, tile, lds); break;
-        case 7: top_pass_body<INV, FROM_LIMBS, 7>(A, pr, tile, lds); break;
-        default: top_pass_body<INV, FROM_LIMBS, 8>(A, pr, tile, lds); break;
     }
 }
 
@@ -225,7 +213,7 @@ struct MidArgs {
 // SEL: pick the per-prime pointers with selects (an argument struct built
 // in registers, batched kernels) instead of indexing the kernel-argument
 // struct directly (single-product kernel, where that keeps fewer registers).
-template <int CB, bool FROM_LIMBS, bool SQR, bool FWD_ONLY, bool SEL>
+template <int CB, bool FROM_LIMBS, bool SQR, bool FWD_ONLY, bool SEL, bool LAZY>
 __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u64* lds, bool sqr_rt = false) {
     const Prime P = make_prime(PRIME_K[0] * (pr == 0) + PRIME_K[1] * (pr == 1) + PRIME_K[2] * (pr == 2));
     constexpr int LO_IO = CB - 4;
@@ -253,7 +241,7 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
         else a[j] = fa_()[x];
     });
     if constexpr (FWD_ONLY) {
-        fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0>(P, a, lds, t, chunk, twf_());
+        fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0, LAZY>(P, a, lds, t, chunk, twf_());
         if constexpr (LO_F != LO_IO) xchg<CB, LO_F, LO_IO>(a, lds, t);
         CFor<0, 16>::run([&](auto jc) {
             constexpr int j = decltype(jc)::value;
@@ -262,7 +250,7 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
         return;
     } else {
         if (!A.a_done) {
-            fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0>(P, a, lds, t, chunk, twf_());
+            fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0, LAZY>(P, a, lds, t, chunk, twf_());
             if constexpr (LO_F != 0) xchg<CB, LO_F, 0>(a, lds, t);
         } else {
             xchg<CB, LO_IO, 0>(a, lds, t);
@@ -275,7 +263,7 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
                 else b[j] = fb_()[x];
             });
             if (!A.b_done) {
-                fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0>(P, b, lds, t, chunk, twf_());
+                fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0, LAZY>(P, b, lds, t, chunk, twf_());
                 if constexpr (LO_F != 0) xchg<CB, LO_F, 0>(b, lds, t);
             } else {
                 xchg<CB, LO_IO, 0>(b, lds, t);
@@ -287,9 +275,9 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
         const u64* twg = twf_() + (base >> 2) + 4 * (u64)t;
         CFor<0, 4>::run([&](auto gc) {
             constexpr int g = decltype(gc)::value, j = 4 * g;
-            base_mul4(P, a[j], a[j + 1], a[j + 2], a[j + 3], b[j], b[j + 1], b[j + 2], b[j + 3], twg[g]);
+            base_mul4<LAZY>(P, a[j], a[j + 1], a[j + 2], a[j + 3], b[j], b[j + 1], b[j + 2], b[j + 3], twg[g]);
         });
-        inv_rec<CB, 1, 1, CB - 1, 0>(P, a, lds, t, chunk, twi_());
+        inv_rec<CB, 1, 1, CB - 1, 0, LAZY>(P, a, lds, t, chunk, twi_());
         constexpr int LO_I = inv_window(CB - 1, 1, 0, CB);
         if constexpr (LO_I != LO_IO) xchg<CB, LO_I, LO_IO>(a, lds, t);
         CFor<0, 16>::run([&](auto jc) {

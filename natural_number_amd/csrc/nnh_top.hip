@@ -1,97 +1,22 @@
-// nnh_top.hip — strided "top" passes of the transform (levels h >= 4096):
-// forward (fused with the limb load on the first pass) and inverse, for one
-// product or a bucket of a batch.
-#include "nnh_tile.hpp"
+// nnh_top.hip — top passes for p2 (strict butterflies) and the dispatch of
+// one top pass over all three primes.
+#include "nnh_top_kernels.hpp"
 
 namespace nnh {
 
-template <bool INV, bool FROM_LIMBS, int NLEV>
-__global__ __launch_bounds__(256) void k_top_pass(TopArgs A) {
-    __shared__ u64 lds[4096 + 256];
-    top_pass_body<INV, FROM_LIMBS, NLEV>(A, blockIdx.y, blockIdx.x, lds);
-}
-
-
-// Batched: persistent grid-stride loop over (job, tile, prime) items of a
-// bucket of equal transform length.
-template <bool INV, bool FROM_LIMBS, int NLEV>
-__global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__ jobs, u32 njobs, int operand,
-                                                        TopBatchParams P) {
-    __shared__ u64 lds[4096 + 256];
-    const u64 tpj = P.L >> 12;
-    const u64 items = (u64)njobs * tpj * 3;
-    for (u64 it = blockIdx.x; it < items; it += gridDim.x) {
-        const u32 pr = (u32)(it % 3);
-        const u64 r = it / 3;
-        const u64 tile = r % tpj;
-        const BJob& J = jobs[r / tpj];
-        if (operand && J.sqr) continue;  // uniform per work-group
-        TopArgs A;
-        u64* base = operand ? J.fb : J.fa;
-        for (int i = 0; i < 3; i++) {
-            A.data[i] = base + (u64)i * P.L;
-            A.tw[i] = P.tw[i];
-            A.alpha_m[i] = P.alpha_m[i];
-        }
-        A.limbs = operand ? J.b : J.a;
-        A.an = operand ? J.bn : J.an;
-        A.g_lo = P.g_lo;
-        A.nlev = P.nlev;
-        top_pass_body<INV, FROM_LIMBS, NLEV>(A, pr, tile, lds);
-        __syncthreads();
-    }
-}
-
-
-// Host dispatch on the pass depth (1..8 levels), known per launch.
-template <bool INV, bool FL>
-static void top_single(const TopArgs& A, u32 nlev, dim3 grid, hipStream_t st) {
-    switch (nlev) {
-#define NNH_TP(n) \
-    case n: hipLaunchKernelGGL((k_top_pass<INV, FL, n>), grid, dim3(256), 0, st, A); break;
-        NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
-        default: hipLaunchKernelGGL((k_top_pass<INV, FL, 8>), grid, dim3(256), 0, st, A); break;
-#undef NNH_TP
-    }
-}
-template <bool INV, bool FL>
-static void top_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, dim3 grid, hipStream_t st) {
-    switch (P.nlev) {
-#define NNH_TP(n)                                                                                              \
-    case n:                                                                                                    \
-        hipLaunchKernelGGL((k_top_pass_batch<INV, FL, n>), grid, dim3(256), 0, st, jobs, njobs, operand, P); \
-        break;
-        NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
-        default: hipLaunchKernelGGL((k_top_pass_batch<INV, FL, 8>), grid, dim3(256), 0, st, jobs, njobs, operand, P);
-            break;
-#undef NNH_TP
-    }
-}
+void launch_top_pass_part(const TopPassDesc& d, bool lazy, hipStream_t st);
+void launch_top_pass_batch_part(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
+                                bool from_limbs, bool lazy, hipStream_t st);
 
 void launch_top_pass(const TopPassDesc& D, hipStream_t st) {
-    TopArgs A;
-    for (int i = 0; i < 3; i++) {
-        A.data[i] = D.data[i];
-        A.tw[i] = D.tw[i];
-        A.alpha_m[i] = D.alpha_m[i];
-    }
-    A.limbs = D.limbs;
-    A.an = D.an;
-    A.g_lo = D.g_lo;
-    A.nlev = D.nlev;
-    dim3 grid((unsigned)(D.L >> 12), 3);
-    if (D.inverse) top_single<true, false>(A, D.nlev, grid, st);
-    else if (D.limbs) top_single<false, true>(A, D.nlev, grid, st);
-    else top_single<false, false>(A, D.nlev, grid, st);
+    launch_top_pass_part(D, true, st);
+    top_pass_primes<false>(D, st);
 }
 
 void launch_top_pass_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
                            bool from_limbs, hipStream_t st) {
-    u64 items = (u64)njobs * (P.L >> 12) * 3;
-    dim3 grid(persist_grid(items, 4096));
-    if (inverse) top_batch<true, false>(jobs, njobs, operand, P, grid, st);
-    else if (from_limbs) top_batch<false, true>(jobs, njobs, operand, P, grid, st);
-    else top_batch<false, false>(jobs, njobs, operand, P, grid, st);
+    launch_top_pass_batch_part(jobs, njobs, operand, P, inverse, from_limbs, true, st);
+    top_pass_batch_primes<false>(jobs, njobs, operand, P, inverse, from_limbs, st);
 }
 
 }  // namespace nnh

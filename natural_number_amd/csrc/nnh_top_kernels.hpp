@@ -1,0 +1,105 @@
+// nnh_top_kernels.hpp — strided "top" passes of the transform (levels
+// h >= 4096): forward (fused with the limb load on the first pass) and
+// inverse, for one product or a bucket of a batch.  LAZY kernels serve the
+// primes p0 and p1 (grid dimension y = 2, lazy butterflies); the others
+// serve p2.  Each translation unit instantiates one LAZY value.
+#pragma once
+#include "nnh_tile.hpp"
+
+namespace nnh {
+
+template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY>
+__global__ __launch_bounds__(256) void k_top_pass(TopArgs A, u32 pr0) {
+    __shared__ u64 lds[4096 + 256];
+    top_pass_body<INV, FROM_LIMBS, NLEV, LAZY>(A, pr0 + blockIdx.y, blockIdx.x, lds);
+}
+
+// Batched: persistent grid-stride loop over (job, tile, prime) items of a
+// bucket of equal transform length, primes pr0 .. pr0+npr-1.
+template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY>
+__global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__ jobs, u32 njobs, int operand,
+                                                        TopBatchParams P, u32 pr0, u32 npr) {
+    __shared__ u64 lds[4096 + 256];
+    const u64 tpj = P.L >> 12;
+    const u64 items = (u64)njobs * tpj * npr;
+    for (u64 it = blockIdx.x; it < items; it += gridDim.x) {
+        const u32 pr = pr0 + (u32)(it % npr);
+        const u64 r = it / npr;
+        const u64 tile = r % tpj;
+        const BJob& J = jobs[r / tpj];
+        if (operand && J.sqr) continue;  // uniform per work-group
+        TopArgs A;
+        u64* base = operand ? J.fb : J.fa;
+        for (int i = 0; i < 3; i++) {
+            A.data[i] = base + (u64)i * P.L;
+            A.tw[i] = P.tw[i];
+            A.alpha_m[i] = P.alpha_m[i];
+        }
+        A.limbs = operand ? J.b : J.a;
+        A.an = operand ? J.bn : J.an;
+        A.g_lo = P.g_lo;
+        A.nlev = P.nlev;
+        top_pass_body<INV, FROM_LIMBS, NLEV, LAZY>(A, pr, tile, lds);
+        __syncthreads();
+    }
+}
+
+// Host dispatch on the pass depth (1..8 levels), known per launch.
+template <bool INV, bool FL, bool LAZY>
+static void top_single(const TopArgs& A, u32 nlev, dim3 grid, u32 pr0, hipStream_t st) {
+    switch (nlev) {
+#define NNH_TP(n) \
+    case n: hipLaunchKernelGGL((k_top_pass<INV, FL, n, LAZY>), grid, dim3(256), 0, st, A, pr0); break;
+        NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
+        default: hipLaunchKernelGGL((k_top_pass<INV, FL, 8, LAZY>), grid, dim3(256), 0, st, A, pr0); break;
+#undef NNH_TP
+    }
+}
+template <bool INV, bool FL, bool LAZY>
+static void top_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, dim3 grid, u32 pr0, u32 npr,
+                      hipStream_t st) {
+    switch (P.nlev) {
+#define NNH_TP(n)                                                                                     \
+    case n:                                                                                           \
+        hipLaunchKernelGGL((k_top_pass_batch<INV, FL, n, LAZY>), grid, dim3(256), 0, st, jobs, njobs, \
+                           operand, P, pr0, npr);                                                     \
+        break;
+        NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
+        default: hipLaunchKernelGGL((k_top_pass_batch<INV, FL, 8, LAZY>), grid, dim3(256), 0, st, jobs, njobs,
+                                    operand, P, pr0, npr);
+            break;
+#undef NNH_TP
+    }
+}
+
+template <bool LAZY>
+static void top_pass_primes(const TopPassDesc& D, hipStream_t st) {
+    TopArgs A;
+    for (int i = 0; i < 3; i++) {
+        A.data[i] = D.data[i];
+        A.tw[i] = D.tw[i];
+        A.alpha_m[i] = D.alpha_m[i];
+    }
+    A.limbs = D.limbs;
+    A.an = D.an;
+    A.g_lo = D.g_lo;
+    A.nlev = D.nlev;
+    const u32 pr0 = LAZY ? 0 : 2, npr = LAZY ? 2 : 1;
+    dim3 grid((unsigned)(D.L >> 12), npr);
+    if (D.inverse) top_single<true, false, LAZY>(A, D.nlev, grid, pr0, st);
+    else if (D.limbs) top_single<false, true, LAZY>(A, D.nlev, grid, pr0, st);
+    else top_single<false, false, LAZY>(A, D.nlev, grid, pr0, st);
+}
+
+template <bool LAZY>
+static void top_pass_batch_primes(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
+                                  bool from_limbs, hipStream_t st) {
+    const u32 pr0 = LAZY ? 0 : 2, npr = LAZY ? 2 : 1;
+    u64 items = (u64)njobs * (P.L >> 12) * npr;
+    dim3 grid(persist_grid(items, 4096));
+    if (inverse) top_batch<true, false, LAZY>(jobs, njobs, operand, P, grid, pr0, npr, st);
+    else if (from_limbs) top_batch<false, true, LAZY>(jobs, njobs, operand, P, grid, pr0, npr, st);
+    else top_batch<false, false, LAZY>(jobs, njobs, operand, P, grid, pr0, npr, st);
+}
+
+}  // namespace nnh

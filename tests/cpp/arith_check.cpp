@@ -48,6 +48,33 @@ static int check_prime(std::mt19937_64& rng, int iters, int pi) {
         u64 gx = rng() % p, gy = rng() % p, ex = gx, ey = gy;
         gs_bfly(P, ex, ey, wm);
         if (ex != (gx + gy) % p || ey != (u64)((u128)((gx + p - gy) % p) * w % p)) bad++;
+        if (lazy_prime(pi)) {
+            // forward lazy butterfly: inputs and outputs in [0, 4p), boundary inputs included
+            u64 lx = it % 5 == 0 ? 4 * p - 1 : rng() % (4 * p), ly = it % 7 == 0 ? 4 * p - 1 : rng() % (4 * p);
+            u64 lcx = lx, lcy = ly;
+            ct_bfly_lazy(P, lcx, lcy, wm);
+            u64 lt = (u64)((u128)(ly % p) * w % p);
+            if (lcx >= 4 * p || lcy >= 4 * p || lcx % p != (lx % p + lt) % p || lcy % p != (lx % p + p - lt) % p)
+                bad++;
+            // inverse lazy butterfly: inputs and outputs in [0, 2p)
+            u64 ix = it % 5 == 1 ? 2 * p - 1 : rng() % (2 * p), iy = it % 7 == 1 ? 2 * p - 1 : rng() % (2 * p);
+            u64 lex = ix, ley = iy;
+            gs_bfly_lazy(P, lex, ley, wm);
+            if (lex >= 2 * p || ley >= 2 * p || lex % p != (ix + iy) % p ||
+                ley % p != (u64)((u128)((ix % p + p - iy % p) % p) * w % p))
+                bad++;
+            // base multiply: lazy inputs in [0, 4p) give the canonical result of reduced inputs
+            u64 in[8], red[8];
+            for (int q = 0; q < 8; q++) {
+                in[q] = (it + q) % 9 == 0 ? 4 * p - 1 : rng() % (4 * p);
+                red[q] = in[q] % p;
+            }
+            u64 l0 = in[0], l1 = in[1], l2 = in[2], l3 = in[3];
+            u64 s0 = red[0], s1 = red[1], s2 = red[2], s3 = red[3];
+            base_mul4<true>(P, l0, l1, l2, l3, in[4], in[5], in[6], in[7], wm);
+            base_mul4<false>(P, s0, s1, s2, s3, red[4], red[5], red[6], red[7], wm);
+            if (l0 != s0 || l1 != s1 || l2 != s2 || l3 != s3) bad++;
+        }
     }
     if (redc(P, 0, 0) != 0) bad++;
     std::printf("prime k=%u: %d failures in %d iterations\n", K, bad, iters);
