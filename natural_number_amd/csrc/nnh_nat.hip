@@ -143,24 +143,52 @@ __global__ void k_shr_bits(u64* out, const u64* a, u64 n, u32 b) {
 __global__ void k_set_word(u64* p, u64 v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
+// Highest qualifying index, scanned top-down in chunks: a block whose chunk
+// lies below an index already found exits at once, so normalising a number
+// whose top word is nonzero reads one chunk.
+constexpr u64 FIND_CHUNK = 2048;
+
 __global__ __launch_bounds__(256) void k_find(const u64* a, const u64* b, u64 n, int mode, u64* res3) {
-    unsigned long long best = 0;
-    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) {
-        u64 x = a[k];
-        bool hit = mode == 0 ? x != 0 : (mode == 1 ? x != ~0ull : x != b[k]);
-        if (hit) best = k + 1;
+    __shared__ u32 skip;
+    for (u64 c = blockIdx.x; c * FIND_CHUNK < n; c += gridDim.x) {
+        const u64 hi = n - c * FIND_CHUNK;
+        const u64 lo = hi > FIND_CHUNK ? hi - FIND_CHUNK : 0;
+        if (threadIdx.x == 0)
+            skip = __hip_atomic_load((unsigned long long*)res3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > hi;
+        __syncthreads();
+        if (skip) return;  // every later chunk of this block is lower still
+        unsigned long long best = 0;
+        for (u64 k = lo + threadIdx.x; k < hi; k += blockDim.x) {
+            u64 x = a[k];
+            bool hit = mode == 0 ? x != 0 : (mode == 1 ? x != ~0ull : x != b[k]);
+            if (hit) best = k + 1;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(best, off);
+            best = o > best ? o : best;
+        }
+        if ((threadIdx.x & 63) == 0 && best) atomicMax((unsigned long long*)res3, best);
+        __syncthreads();
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(best, off);
-        best = o > best ? o : best;
-    }
-    if ((threadIdx.x & 63) == 0 && best) atomicMax((unsigned long long*)res3, best);
 }
-__global__ void k_fetch(u64* res3, const u64* a, const u64* b) {
+// Publish {index, a[index-1], b[index-1]} to host-visible memory with a
+// sequence number, and clear the device result for the next search.
+__global__ void k_fetch(u64* res3, const u64* a, const u64* b, u64* hout, u64 seq) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     u64 i = res3[0];
-    res3[1] = i ? a[i - 1] : 0;
-    res3[2] = (i && b) ? b[i - 1] : 0;
+    hout[0] = i;
+    hout[1] = i ? a[i - 1] : 0;
+    hout[2] = (i && b) ? b[i - 1] : 0;
+    res3[0] = 0;
+    __threadfence_system();
+    hout[3] = seq;
+}
+// Publish up to three words of p to host-visible memory with a sequence number.
+__global__ void k_peek(const u64* p, u32 n, u64* hout, u64 seq) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (u32 i = 0; i < n; i++) hout[i] = p[i];
+    __threadfence_system();
+    hout[3] = seq;
 }
 
 static unsigned grid_for(u64 n) { return persist_grid((n + 255) / 256, 4096); }
@@ -178,10 +206,14 @@ void launch_shr_bits(u64* out, const u64* a, u64 n, u32 b, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_shr_bits, dim3(grid_for(n)), dim3(256), 0, st, out, a, n, b);
 }
 void launch_set_word(u64* p, u64 v, hipStream_t st) { hipLaunchKernelGGL(k_set_word, dim3(1), dim3(64), 0, st, p, v); }
-void launch_find(const u64* a, const u64* b, u64 n, int mode, u64* res3, hipStream_t st) {
-    launch_set_word(res3, 0, st);
-    if (n) hipLaunchKernelGGL(k_find, dim3(persist_grid((n + 255) / 256, 2048)), dim3(256), 0, st, a, b, n, mode, res3);
-    hipLaunchKernelGGL(k_fetch, dim3(1), dim3(64), 0, st, res3, a, b);
+void launch_find(const u64* a, const u64* b, u64 n, int mode, u64* res3, u64* hout, u64 seq, hipStream_t st) {
+    if (n)
+        hipLaunchKernelGGL(k_find, dim3(persist_grid((n + FIND_CHUNK - 1) / FIND_CHUNK, 1024)), dim3(256), 0, st, a, b,
+                           n, mode, res3);
+    hipLaunchKernelGGL(k_fetch, dim3(1), dim3(64), 0, st, res3, a, b, hout, seq);
+}
+void launch_peek(const u64* p, u32 n, u64* hout, u64 seq, hipStream_t st) {
+    hipLaunchKernelGGL(k_peek, dim3(1), dim3(64), 0, st, p, n, hout, seq);
 }
 
 // ---------------------------------------------------------------------------

@@ -37,9 +37,12 @@ void launch_shl_bits(u64* out, const u64* a, u64 n, u32 b, hipStream_t st);
 // out[0..n) = a >> b (1 <= b <= 63).
 void launch_shr_bits(u64* out, const u64* a, u64 n, u32 b, hipStream_t st);
 // Highest index k < n with: mode 0 a[k] != 0, mode 1 a[k] != ~0,
-// mode 2 a[k] != b[k].  res3 (device, 3 words) receives {k+1 or 0,
-// a[k], b[k]} (zeros when no index qualifies).
-void launch_find(const u64* a, const u64* b, u64 n, int mode, u64* res3, hipStream_t st);
+// mode 2 a[k] != b[k].  hout (host-visible, 4 words) receives {k+1 or 0,
+// a[k], b[k], seq}, seq written last; res (device word, zero on entry) is
+// the search's accumulator and is cleared again for the next search.
+void launch_find(const u64* a, const u64* b, u64 n, int mode, u64* res, u64* hout, u64 seq, hipStream_t st);
+// hout[0..n) = p[0..n) (n <= 3), then hout[3] = seq.
+void launch_peek(const u64* p, u32 n, u64* hout, u64 seq, hipStream_t st);
 void launch_set_word(u64* p, u64 v, hipStream_t st);
 // q[0..n) = a / d0 and *rem = a mod d0, where d = d0 << l has its top bit
 // set and v = floor((2^128 - 1) / d) - 2^64.

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <bit>
 #include <chrono>
 #include <cmath>
@@ -123,7 +124,8 @@ class Ctx {
    public:
     explicit Ctx(hipStream_t s) : st(s) {
         hipchk(hipHostMalloc((void**)&h_, 4096, hipHostMallocDefault), "pinned allocation");
-        res_ = alloc(8);
+        res_ = zeros(8);
+        mailbox();
     }
     ~Ctx() {
         (void)hipStreamSynchronize(st);
@@ -171,20 +173,52 @@ class Ctx {
         hipchk(hipGetLastError(), "launch");
         hipchk(hipStreamSynchronize(st), "synchronise");
     }
+    // Small readbacks: a one-thread kernel writes the words and a sequence
+    // number into pinned, host-coherent memory; the host polls for the
+    // number (short waits), falling back to a stream synchronisation.
+    struct Mailbox {
+        u64* host = nullptr;
+        u64* dev = nullptr;
+        u64 seq = 0;
+    };
+    static Mailbox& mailbox() {
+        static Mailbox m;  // process lifetime: device-written host memory is never freed
+        if (!m.host) {
+            hipchk(hipHostMalloc((void**)&m.host, 64, hipHostMallocMapped | hipHostMallocCoherent), "mailbox");
+            hipchk(hipHostGetDevicePointer((void**)&m.dev, m.host, 0), "mailbox");
+            std::memset(m.host, 0, 64);
+        }
+        return m;
+    }
+    const u64* wait_mail(u64 seq) {
+        Mailbox& m = mailbox();
+        volatile u64* flag = m.host + 3;
+        auto t0 = Clock::now();
+        for (u32 spin = 0; *flag != seq; spin++)
+            if ((spin & 1023) == 1023 && ms_since(t0) > 0.5) {
+                sync();
+                if (*flag != seq) raise(NNH_ERR_HIP, "readback did not arrive");
+                break;
+            }
+        std::atomic_thread_fence(std::memory_order_acquire);
+        volatile const u64* v = m.host;
+        for (int i = 0; i < 3; i++) mail_[i] = v[i];
+        return mail_;
+    }
     u64 read(const u64* p) {
-        hipchk(hipMemcpyAsync(h_, p, sizeof(u64), hipMemcpyDeviceToHost, st), "read");
-        sync();
-        return h_[0];
+        Mailbox& m = mailbox();
+        launch_peek(p, 1, m.dev, ++m.seq, st);
+        return wait_mail(m.seq)[0];
     }
     u64 top(const DN& x) { return read(x.p + x.n - 1); }
     // 1 + highest qualifying index (0: none); wa/wb receive the words there.
     u64 find(const u64* a, const u64* b, u64 n, int mode, u64* wa = nullptr, u64* wb = nullptr) {
-        launch_find(a, b, n, mode, res_.p, st);
-        hipchk(hipMemcpyAsync(h_, res_.p, 3 * sizeof(u64), hipMemcpyDeviceToHost, st), "find");
-        sync();
-        if (wa) *wa = h_[1];
-        if (wb) *wb = h_[2];
-        return h_[0];
+        Mailbox& m = mailbox();
+        launch_find(a, b, n, mode, res_.p, m.dev, ++m.seq, st);
+        const u64* r = wait_mail(m.seq);
+        if (wa) *wa = r[1];
+        if (wb) *wb = r[2];
+        return r[0];
     }
     // natural.h:82-93; returns the top word.
     u64 normalize(DN& x) {
@@ -201,12 +235,13 @@ class Ctx {
     // least B^(n-2) unless an operand was zero, so the top two words decide.
     u64 normalize_top(DN& x) {
         if (x.n < 2) return read(x.p);
-        hipchk(hipMemcpyAsync(h_, x.p + x.n - 2, 2 * sizeof(u64), hipMemcpyDeviceToHost, st), "read");
-        sync();
-        if (h_[1]) return h_[1];
-        if (h_[0]) {
+        Mailbox& m = mailbox();
+        launch_peek(x.p + x.n - 2, 2, m.dev, ++m.seq, st);
+        const u64* r = wait_mail(m.seq);
+        if (r[1]) return r[1];
+        if (r[0]) {
             x.n -= 1;
-            return h_[0];
+            return r[0];
         }
         return normalize(x);
     }
@@ -366,6 +401,7 @@ class Ctx {
 
    private:
     u64* h_ = nullptr;
+    u64 mail_[3] = {0, 0, 0};
     DN res_;
     AddJob* d_jobs_ = nullptr;
     AddJob* h_jobs_ = nullptr;
