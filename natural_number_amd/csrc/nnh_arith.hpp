@@ -3,8 +3,9 @@
 // The primes are the reference's p_i = k_i * 2^56 + 1 with k = 27, 58, 87
 // (ntt_info.h:69-73).  Device code keeps residues in the ordinary domain and
 // multiplies by constants held in Montgomery form (R = 2^64).  Because
-// -p^-1 mod 2^64 = k*2^56 - 1, the Montgomery reduction needs one 32-bit
-// multiply for the quotient m and two v_mad_u64_u32 for m*p.
+// p^-1 mod 2^64 = 1 - k*2^56, the Montgomery quotient m differs from T's low
+// word only in its top byte, and the reduction needs three 32x32 multiplies
+// and no carry flags.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -46,18 +47,18 @@ __host__ __device__ constexpr bool lazy_prime(int i) { return i < 2; }
 __host__ __device__ constexpr Prime prime_of(int i) { return make_prime(PRIME_K[i]); }
 
 // Montgomery reduction for p = k*2^56+1, R = 2^64: returns T * 2^-64 mod p
-// in [0, 2p) for T = hi:lo < p * 2^64.
-//   m = T_lo * (k*2^56 - 1) mod 2^64 = ((T_lo*k mod 2^8) << 56) - T_lo
-//   (T + m*p) / 2^64 = T_hi + (e + m*k) / 2^8,  e = (T_lo + m) >> 56
-// and T_lo + m = byte*2^56 + 2^64*[T_lo > byte*2^56].
+// in (0, 2p) for T = hi:lo < p * 2^64.
+//   m = T_lo * p^-1 mod 2^64 = T_lo - (T_lo*k mod 2^8) * 2^56, so T - m p is
+//   a multiple of 2^64 whose low word cancels exactly: (T - m p) / 2^64 =
+//   T_hi - N with N = floor(m p / 2^64) in [0, p).  With m = mh*2^32 + l:
+//   m p = m + m k 2^56, N = mh*k*2^24 + floor((l*k + (mh >> 24)) / 2^8).
+// The result T_hi - N + p lies in (0, 2p) because T_hi < p.
 __host__ __device__ __forceinline__ u64 redc(const Prime& P, u64 lo, u64 hi) {
-    u32 byte24 = lo32(lo) * P.k24;                   // (T_lo*k mod 2^8) << 24
-    u64 top = (u64)byte24 << 32;                     // byte * 2^56
-    u64 m = top - lo;
-    u32 wrap = lo > top ? 1u : 0u;                   // carry of T_lo + m
-    u64 t = (u64)lo32(m) * P.k + (byte24 >> 24);     // < 2^40, low 8 bits cancel with e
-    u64 r = (u64)hi32(m) * P.k24 + (t >> 8);
-    return hi + r + wrap;
+    u32 l = lo32(lo);
+    u32 mh = hi32(lo) - l * P.k24;                 // high word of m
+    u64 s = (u64)l * P.k + (mh >> 24);             // < 2^40
+    u64 N = (u64)mh * P.k24 + (u32)(s >> 8);
+    return hi + P.p - N;
 }
 
 __host__ __device__ __forceinline__ u64 mont(const Prime& P, u64 a, u64 b) {

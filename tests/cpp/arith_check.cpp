@@ -76,7 +76,14 @@ static int check_prime(std::mt19937_64& rng, int iters, int pi) {
             if (l0 != s0 || l1 != s1 || l2 != s2 || l3 != s3) bad++;
         }
     }
-    if (redc(P, 0, 0) != 0) bad++;
+    // reduction edge cases: zero words, all-ones words, top byte patterns of the quotient
+    const u64 lows[] = {0, 1, 0xffffffffull, 1ull << 32, 0xff00000000000000ull, ~0ull, 0x00ffffffffffffffull,
+                        0x8000000080000000ull};
+    for (u64 lo : lows)
+        for (u64 hi : {(u64)0, (u64)1, p - 1, p / 2}) {
+            u64 r = redc(P, lo, hi);
+            if (r == 0 || r >= 2 * p || r % p != expect((u128)hi << 64 | lo)) bad++;
+        }
     std::printf("prime k=%u: %d failures in %d iterations\n", K, bad, iters);
     return bad;
 }
