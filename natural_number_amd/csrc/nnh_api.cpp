@@ -14,6 +14,7 @@
 #include <bit>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -581,12 +582,14 @@ static int stage_host(Device& D, u64 words) { return ensure(D.stage, words); }
 struct Bucket {
     int s;
     u32 first, count;
+    u32 own_first, own_count;  // forward-only entries of the bucket's shared B transforms
 };
 
 }  // namespace nnh
 
 struct nnh_batch {
-    std::vector<nnh::BJob> jobs;
+    std::vector<nnh::BJob> jobs;  // the requests' jobs, then the forward-only entries
+    nnh::u32 n_req_jobs = 0;
     std::vector<nnh::Bucket> buckets;
     nnh::BJob* d_jobs = nullptr;
     nnh::u32* d_sj = nullptr;
@@ -645,7 +648,8 @@ static void batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
             mp.alpha_m[i] = pl.alpha_m[i];
         }
         mp.L = pl.L;
-        launch_mid_batch(jobs, bk.count, mp, pl.cb, pl.ntop == 0, st);
+        if (bk.own_count) launch_mid_batch(B->d_jobs + bk.own_first, bk.own_count, mp, pl.cb, pl.ntop == 0, true, st);
+        launch_mid_batch(jobs, bk.count, mp, pl.cb, pl.ntop == 0, false, st);
         for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
             for (int i = 0; i < 3; i++) tp.tw[i] = D.twi[i];
             tp.g_lo = (u32)it->first;
@@ -653,7 +657,7 @@ static void batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
             launch_top_pass_batch(jobs, bk.count, 0, tp, true, false, st);
         }
     }
-    launch_crt_batch(B->d_jobs, (u32)B->jobs.size(), B->ntiles, B->d_tiles, B->d_tiles + B->ntiles, D.crt, st);
+    launch_crt_batch(B->d_jobs, B->n_req_jobs, B->ntiles, B->d_tiles, B->d_tiles + B->ntiles, D.crt, st);
 }
 
 static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch** out) {
@@ -681,8 +685,11 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
         B->max_s = std::max(B->max_s, s);
     }
     std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.s < y.s; });
-    // workspace offsets
+    // workspace offsets (pointers hold word offsets into the workspace until rebased below)
     u64 ws = 0, tiles = 0, cols = 0;
+    // B operands repeated within a bucket (same pointer and size) are
+    // transformed once: the first such job owns the transform, later ones read it
+    std::map<std::pair<const u64*, u64>, u32> owner_of;
     for (const Item& it : items) {
         const IReq& r = reqs[it.idx];
         BJob J{};
@@ -709,14 +716,47 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
             J.sqr = sq;
             J.n_res = std::min<u64>(r.an + r.bn - 1, L);
             J.fa = (u64*)(uintptr_t)(ws * 8);
-            J.fb = (u64*)(uintptr_t)((ws + 3 * L) * 8);
-            ws += 6 * L;
-            if (B->buckets.empty() || B->buckets.back().s != it.s)
-                B->buckets.push_back({it.s, (u32)B->jobs.size(), 0});
+            for (int i = 0; i < 3; i++) J.r[i] = (const u64*)(uintptr_t)((ws + (u64)i * L) * 8);
+            ws += 3 * L;
+            if (B->buckets.empty() || B->buckets.back().s != it.s) {
+                B->buckets.push_back({it.s, (u32)B->jobs.size(), 0, 0, 0});
+                owner_of.clear();
+            }
             B->buckets.back().count++;
+            auto key = std::make_pair(r.b, r.bn);
+            auto own = sq ? owner_of.end() : owner_of.find(key);
+            if (own != owner_of.end()) {
+                BJob& O = B->jobs[own->second];
+                O.share = 1;
+                J.share = 2;
+                J.fb = O.fb;
+            } else {
+                J.fb = (u64*)(uintptr_t)(ws * 8);
+                ws += 3 * L;
+                if (!sq) owner_of[key] = (u32)B->jobs.size();
+            }
         }
         if (it.s == 0) B->n_school++;
         B->jobs.push_back(J);
+    }
+    // forward-only entries: transform each shared B once into its owner's fb
+    B->n_req_jobs = (u32)B->jobs.size();
+    for (Bucket& bk : B->buckets) {
+        bk.own_first = (u32)B->jobs.size();
+        for (u32 j = bk.first; j < bk.first + bk.count; j++) {
+            const BJob& O = B->jobs[j];
+            if (O.share != 1) continue;
+            BJob F{};
+            F.a = O.b;
+            F.an = O.bn;
+            F.b = O.b;
+            F.bn = O.bn;
+            F.fa = O.fb;
+            F.fb = O.fb;
+            F.n_res = O.n_res;
+            B->jobs.push_back(F);
+        }
+        bk.own_count = (u32)B->jobs.size() - bk.own_first;
     }
     B->ncols = cols;
     B->ntiles = tiles;
@@ -750,14 +790,15 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
             return rc;
         }
     }
-    for (BJob& J : B->jobs) {
+    for (u32 j = 0; j < B->jobs.size(); j++) {
+        BJob& J = B->jobs[j];
         J.fa = B->ws + (uintptr_t)J.fa / 8;
-        if (!J.school) J.fb = B->ws + (uintptr_t)J.fb / 8;
         if (J.school) {
             for (int i = 0; i < 3; i++) J.r[i] = J.fa + (u64)i * J.n_res;
         } else {
-            u64 L = (J.fb - J.fa) / 3;
-            for (int i = 0; i < 3; i++) J.r[i] = J.fa + (u64)i * L;
+            J.fb = B->ws + (uintptr_t)J.fb / 8;
+            if (j < B->n_req_jobs)
+                for (int i = 0; i < 3; i++) J.r[i] = B->ws + (uintptr_t)J.r[i] / 8;
             if (J.sqr) J.fb = J.fa;
         }
     }

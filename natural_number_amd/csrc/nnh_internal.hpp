@@ -82,6 +82,7 @@ struct BJob {
     u64 n_add;
     u32 school;       // 1: schoolbook columns
     u32 sqr;          // 1: b == a (single transform)
+    u32 share;        // B transform shared within the batch: 0 no, 1 owner, 2 reader of the owner's fb
 };
 
 struct TopBatchParams {
@@ -100,7 +101,9 @@ struct MidBatchParams {
 
 void launch_top_pass_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
                            bool from_limbs, hipStream_t st);
-void launch_mid_batch(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs, hipStream_t st);
+// fwd_only: the jobs are forward-only entries that store shared B transforms.
+void launch_mid_batch(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs, bool fwd_only,
+                      hipStream_t st);
 void launch_school_batch(const BJob* jobs, const u32* sj, u32 nsj, u64 ncols, hipStream_t st);
 void launch_crt_batch(const BJob* jobs, u32 njobs, u64 ntiles, u8* tile_fn, u8* tile_cin, const CrtConsts& C,
                       hipStream_t st);

@@ -14,7 +14,10 @@ __global__ __launch_bounds__(1 << (CB - 4)) void k_mid(MidArgs A, u32 pr0) {
     mid_body<CB, FROM_LIMBS, SQR, FWD_ONLY, false, LAZY>(A, pr, blockIdx.x, lds);
 }
 
-template <int CB, bool FROM_LIMBS, bool LAZY>
+// FWD_ONLY: the jobs are the forward-only entries of shared B transforms
+// (fa = the owner's fb, a = its B limbs); the finished transform is stored.
+// Otherwise a job with share != 0 reads B's finished transform from fb.
+template <int CB, bool FROM_LIMBS, bool FWD_ONLY, bool LAZY>
 __global__ __launch_bounds__(1 << (CB - 4)) void k_mid_batch(const BJob* __restrict__ jobs, u32 njobs,
                                                             MidBatchParams P, u32 pr0, u32 npr) {
     __shared__ u64 lds[(1 << CB) + (1 << (CB - 4))];
@@ -34,13 +37,13 @@ __global__ __launch_bounds__(1 << (CB - 4)) void k_mid_batch(const BJob* __restr
             A.alpha_m[i] = P.alpha_m[i];
         }
         A.la = J.a;
-        A.lb = FROM_LIMBS ? J.b : nullptr;
+        A.lb = FROM_LIMBS && !J.share ? J.b : nullptr;
         A.an = J.an;
         A.bn = J.bn;
         A.a_done = 0;
-        A.b_done = 0;
+        A.b_done = J.share != 0;
         bool sq = J.sqr != 0;
-        mid_body<CB, FROM_LIMBS, false, false, true, LAZY>(A, pr, chunk, lds, sq);
+        mid_body<CB, FROM_LIMBS, false, FWD_ONLY, true, LAZY>(A, pr, chunk, lds, sq);
         __syncthreads();
     }
 }
@@ -84,23 +87,32 @@ static void mid_primes(const MidDesc& D, hipStream_t st) {
 }
 
 template <int CB, bool LAZY>
-static void mid_batch_dispatch(const BJob* jobs, u32 njobs, const MidBatchParams& P, bool from_limbs, hipStream_t st) {
+static void mid_batch_dispatch(const BJob* jobs, u32 njobs, const MidBatchParams& P, bool from_limbs, bool fwd_only,
+                               hipStream_t st) {
     const u32 pr0 = LAZY ? 0 : 2, npr = LAZY ? 2 : 1;
     u64 items = (u64)njobs * (P.L >> CB) * npr;
     dim3 grid(persist_grid(items, 4096)), block(1 << (CB - 4));
-    if (from_limbs) hipLaunchKernelGGL((k_mid_batch<CB, true, LAZY>), grid, block, 0, st, jobs, njobs, P, pr0, npr);
-    else hipLaunchKernelGGL((k_mid_batch<CB, false, LAZY>), grid, block, 0, st, jobs, njobs, P, pr0, npr);
+#define NNH_MB(fl, fo) \
+    hipLaunchKernelGGL((k_mid_batch<CB, fl, fo, LAZY>), grid, block, 0, st, jobs, njobs, P, pr0, npr)
+    if (fwd_only) {
+        if (from_limbs) NNH_MB(true, true);
+        else NNH_MB(false, true);
+    } else {
+        if (from_limbs) NNH_MB(true, false);
+        else NNH_MB(false, false);
+    }
+#undef NNH_MB
 }
 
 template <bool LAZY>
 static void mid_batch_primes(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs,
-                             hipStream_t st) {
+                             bool fwd_only, hipStream_t st) {
     switch (cb) {
-        case 8: mid_batch_dispatch<8, LAZY>(jobs, njobs, P, from_limbs, st); break;
-        case 9: mid_batch_dispatch<9, LAZY>(jobs, njobs, P, from_limbs, st); break;
-        case 10: mid_batch_dispatch<10, LAZY>(jobs, njobs, P, from_limbs, st); break;
-        case 11: mid_batch_dispatch<11, LAZY>(jobs, njobs, P, from_limbs, st); break;
-        default: mid_batch_dispatch<12, LAZY>(jobs, njobs, P, from_limbs, st); break;
+        case 8: mid_batch_dispatch<8, LAZY>(jobs, njobs, P, from_limbs, fwd_only, st); break;
+        case 9: mid_batch_dispatch<9, LAZY>(jobs, njobs, P, from_limbs, fwd_only, st); break;
+        case 10: mid_batch_dispatch<10, LAZY>(jobs, njobs, P, from_limbs, fwd_only, st); break;
+        case 11: mid_batch_dispatch<11, LAZY>(jobs, njobs, P, from_limbs, fwd_only, st); break;
+        default: mid_batch_dispatch<12, LAZY>(jobs, njobs, P, from_limbs, fwd_only, st); break;
     }
 }
 

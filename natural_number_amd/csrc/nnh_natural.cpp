@@ -933,22 +933,23 @@ void calc_pi(Ctx& c, int k, DN& num, DN& den, DN& numc) {
             launch_pi_combine(cn.p, cd.p, cc.p, S, xn.p, xd.p, xc.p, S2, ncomb, c.st);
         } else {
             // natural.h:893-901: the numc_l num_r product first, then
-            // num_l den_r with it as the CRT addend
+            // num_l den_r with it as the CRT addend.  Each batch repeats one
+            // B operand per combination (numc_l, then den_r), whose transform
+            // the batch computes once.
             DN tmp = c.alloc(ncomb * S2);
-            std::vector<IReq> rq(3 * ncomb);
+            std::vector<IReq> rq(2 * ncomb);
             for (u64 j = 0; j < ncomb; j++) {
                 const u64 L = 2 * j * S, R = (2 * j + 1) * S;
-                rq[3 * j] = IReq{tmp.p + j * S2, cc.p + L, S, cn.p + R, S, S2, nullptr, 0};
-                rq[3 * j + 1] = IReq{xd.p + j * S2, cd.p + L, S, cd.p + R, S, S2, nullptr, 0};
-                rq[3 * j + 2] = IReq{xc.p + j * S2, cc.p + L, S, cc.p + R, S, S2, nullptr, 0};
+                rq[2 * j] = IReq{tmp.p + j * S2, cn.p + R, S, cc.p + L, S, S2, nullptr, 0};
+                rq[2 * j + 1] = IReq{xc.p + j * S2, cc.p + R, S, cc.p + L, S, S2, nullptr, 0};
             }
             rcchk(dev_batch(rq.data(), rq.size(), c.st));
-            rq.resize(ncomb);
             for (u64 j = 0; j < ncomb; j++) {
                 const u64 L = 2 * j * S, R = (2 * j + 1) * S;
-                rq[j] = IReq{xn.p + j * S2, cn.p + L, S, cd.p + R, S, S2, tmp.p + j * S2, S2};
+                rq[2 * j] = IReq{xd.p + j * S2, cd.p + L, S, cd.p + R, S, S2, nullptr, 0};
+                rq[2 * j + 1] = IReq{xn.p + j * S2, cn.p + L, S, cd.p + R, S, S2, tmp.p + j * S2, S2};
             }
-            rcchk(dev_batch(rq.data(), ncomb, c.st));
+            rcchk(dev_batch(rq.data(), rq.size(), c.st));
         }
         cn = xn;
         cd = xd;

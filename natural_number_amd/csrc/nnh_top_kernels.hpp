@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__
         const u64 r = it / npr;
         const u64 tile = r % tpj;
         const BJob& J = jobs[r / tpj];
-        if (operand && J.sqr) continue;  // uniform per work-group
+        if (operand && (J.sqr || J.share == 2)) continue;  // uniform per work-group
         TopArgs A;
         u64* base = operand ? J.fb : J.fa;
         for (int i = 0; i < 3; i++) {

@@ -102,3 +102,54 @@ def test_batch_config3_matches_single_path(nnh):
         for (a, bb), got in zip(ops[::7], batched[::7]):
             assert to_int(got) % q == (to_int(a) % q) * (to_int(bb) % q) % q
     b.close()
+
+
+def test_batch_shared_b_transform(nnh, oracle):
+    """Requests that repeat a B operand (same device pointer and size) share
+    one transform of it inside the batch; products stay bit-exact.  Covers
+    chunk-kernel-only scales, scales with top passes, a squaring request on
+    the shared pointer, and B reused as another request's A."""
+    rng = np.random.default_rng(91)
+    groups = [(700, [600, 700, 650, 1, 699]), (3000, [2500, 3000, 2900, 3000]),
+              (40000, [39000, 40000, 20000, 40000, 33333])]
+    host, offs = [], []
+    pos = 0
+
+    def put(x):
+        nonlocal pos
+        host.append(x)
+        offs.append(pos)
+        pos += x.size
+        return offs[-1]
+
+    plan = []
+    for bn, ans in groups:
+        b = rand_limbs(rng, bn)
+        ob = put(b)
+        for an in ans:
+            a = rand_limbs(rng, an)
+            plan.append((put(a), a, ob, b))
+        plan.append((ob, b, ob, b))          # square on the shared pointer
+        c = rand_limbs(rng, bn // 2 + 1)
+        plan.append((ob, b, put(c), c))      # the shared B as an A operand
+    inp = nnh.DeviceWords(pos, np.concatenate(host))
+    n_out = sum(a.size + b.size for _, a, _, b in plan)
+    out = nnh.DeviceWords(n_out)
+    reqs, spans, o = [], [], 0
+    for oa, a, ob, b in plan:
+        reqs.append((out.at(o), inp.at(oa), a.size, inp.at(ob), b.size))
+        spans.append((o, a.size + b.size))
+        o += a.size + b.size
+    zeros = np.zeros(n_out, dtype=np.uint64)
+    for graph in (False, True):
+        say(f"shared-B batch of {len(reqs)} graph={graph}")
+        bt = nnh.Batch(reqs, graph=graph)
+        for rep in range(2):
+            out.upload(zeros)
+            bt.run()
+            nnh.synchronize()
+            got = out.download()
+            for (oa, a, ob, b), (s, n) in zip(plan, spans):
+                exp = oracle.sqr(a) if oa == ob else oracle.mul(a, b)
+                assert np.array_equal(got[s:s + n], exp), (a.size, b.size, graph, rep)
+        bt.close()
