@@ -163,18 +163,33 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
     const u64* __restrict__ tw = sel3(A.tw, pr);
     const u64 alpha = sel3(A.alpha_m, pr);
     u64 v[16];
+    // The limb-loading pass is the first one: its top level (local bit CB-1)
+    // pairs x with x + L/2 in block 0, whose twiddle is 1.  When the operand
+    // fits in the lower half the partner is zero and the level is a copy.
+    // (Uniform per work-group.)
+    const bool top_copy = FROM_LIMBS && !INV && blk == 0 && (d << (NLEV - 1)) >= A.an;
     {
         const u32 yt = lay_t(t, LO_IO);
         CFor<0, 16>::run([&](auto jc) {
-            constexpr u32 jy = (u32)decltype(jc)::value << LO_IO;
+            constexpr int j = decltype(jc)::value;
+            constexpr u32 jy = (u32)j << LO_IO;
             const u32 y = yt | jy;
             const u64 x = xbase + (y & QM) + (u64)(y >> Q) * d;
-            if constexpr (FROM_LIMBS) v[decltype(jc)::value] = load_limb(P, A.limbs, A.an, x, alpha);
-            else v[decltype(jc)::value] = data[x];
+            if constexpr (FROM_LIMBS) {
+                if (j < 8 || !top_copy) v[j] = load_limb(P, A.limbs, A.an, x, alpha);
+            } else {
+                v[j] = data[x];
+            }
         });
     }
     if constexpr (!INV) {
-        fwd_rec<CB, CB - 1, Q, CB - 1, LO_IO, Q, LAZY>(P, v, lds, t, blk, tw);
+        static_assert(LO_IO == CB - 4, "the top level's partner is register j + 8");
+        if (top_copy) {
+            CFor<0, 8>::run([&](auto jc) { v[decltype(jc)::value + 8] = v[decltype(jc)::value]; });
+            fwd_rec<CB, CB - 2, Q, CB - 1, LO_IO, Q, LAZY>(P, v, lds, t, blk, tw);
+        } else {
+            fwd_rec<CB, CB - 1, Q, CB - 1, LO_IO, Q, LAZY>(P, v, lds, t, blk, tw);
+        }
     } else {
         if constexpr (LO_INV0 != LO_IO) xchg<CB, LO_IO, LO_INV0>(v, lds, t);
         inv_rec<CB, Q, Q, CB - 1, LO_INV0, LAZY>(P, v, lds, t, blk, tw);
