@@ -234,6 +234,40 @@ def cost_model_run(n_side, n_held):
     return {"cost_model": ev}
 
 
+def valu_ceiling(nn, kernels, steps):
+    """Butterfly-issue ceiling of the s = 25 product next to the measured kernels.
+
+    The library's register-resident butterfly rate (the same lazy / strict
+    butterfly functions, no memory traffic) gives the fastest the transform
+    levels can run; each kernel's butterfly count divided by that rate is its
+    ideal time.  Butterflies per product: three transforms per prime of 24
+    levels of 2^24 butterflies (the 25th level is the base multiply modulo
+    X^2 -+ w), less the load pass's top level, which is a copy (2 x 2^24 per
+    prime): load 2 x 6, forward 2 x 6, chunk kernel 2 x 11 + 11, inverse 13."""
+    log("butterfly issue ceiling")
+    r_lazy = nn.butterfly_peak(1)
+    r_strict = nn.butterfly_peak(2)
+    half = 1 << 24
+    # ideal seconds per butterfly-level over the three primes (two lazy, one strict)
+    per_level = 2 * half / r_lazy + half / r_strict
+    levels = {"top_pass_load": 2 * 6, "top_pass_fwd": 2 * 6, "mid_fused": 2 * 11 + 11,
+              "top_pass_inv": 13, "crt_carry": 0}
+    out = {"peak_lazy_bfly_per_s": r_lazy, "peak_strict_bfly_per_s": r_strict, "kernels": {}}
+    tot_ideal = tot_meas = 0.0
+    for name, nl in levels.items():
+        k = kernels.get(name)
+        if not k or not nl:
+            continue
+        ideal_ms = nl * per_level * 1e3
+        meas_ms = k["ms_per_step"]
+        out["kernels"][name] = {"butterfly_levels": nl, "ideal_ms": round(ideal_ms, 4),
+                                "measured_ms": meas_ms, "frac": round(ideal_ms / meas_ms, 3)}
+        tot_ideal += ideal_ms
+        tot_meas += meas_ms
+    out["transform_frac"] = round(tot_ideal / tot_meas, 3)
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -341,6 +375,8 @@ def main():
     L = 1 << s
     b_alg = 16 * (an + bn) + 144 * L
 
+    valu = valu_ceiling(nn, kernels, args.steps)
+
     batch = None if args.no_batch else batch_mix(nn, stream)
     e_res = None if args.no_e else e_workload(nn, not args.no_cpu)
     drv = None if args.no_drivers else drivers_workload(nn, not args.no_cpu)
@@ -398,6 +434,7 @@ def main():
             "achieved_gbps": b_alg / (ms * 1e-3) / 1e9,
             "frac": b_alg / (ms * 1e-3) / HBM_PEAK,
         },
+        "valu_ceiling": valu,
         "kernels": kernels,
         "cpu_baseline": cpu,
         "batch_mix": batch,

@@ -163,6 +163,15 @@ uint64_t nnh_threshold_words(void);
 int nnh_kernel_timing(int enable);
 int nnh_kernel_times(int kind, double* total_ms, uint64_t* launches, double* bytes, const char** name);
 
+/* Butterfly issue ceiling (no counterpart in the reference; for
+ * benchmarks): butterflies per second of prime `prime`'s transform
+ * butterfly (the lazy one for primes 0 and 1, the strict one for prime 2)
+ * on register-resident data, `iters` iterations of 16 butterflies per
+ * thread over 2^19 threads.  Transform kernels run the same butterfly
+ * plus loads, stores, LDS exchanges and twiddle reads, so this rate
+ * bounds them from above. */
+int nnh_butterfly_peak(int prime, uint64_t iters, double* butterflies_per_s);
+
 /* Device plumbing (no counterpart in the reference): device memory, copies,
  * streams and events through the library's own HIP runtime, so that a
  * caller needs no second GPU runtime in its process.  Streams are created

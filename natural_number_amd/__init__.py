@@ -66,6 +66,7 @@ def lib():
         "nnh_mul_algo": ([_u64p, _u64p, u64, _u64p, u64, i32, i32], i32),
         "nnh_mul_algo_device_async": ([vp, vp, u64, vp, u64, i32, i32, vp], i32),
         "nnh_time_algo": ([vp, vp, u64, vp, u64, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_float)], i32),
+        "nnh_butterfly_peak": ([i32, u64, ctypes.POINTER(ctypes.c_double)], i32),
         "nnh_mlp_create": ([i32, i32, u64, ctypes.POINTER(vp)], i32),
         "nnh_mlp_destroy": ([vp], i32),
         "nnh_mlp_num_params": ([vp], i32),
@@ -193,6 +194,13 @@ def sqr_device_async(d_out, d_a, an, stream=None):
 
 def kernel_timing(enable):
     _check(lib().nnh_kernel_timing(1 if enable else 0))
+
+
+def butterfly_peak(prime, iters=256):
+    """Register-resident butterflies per second of `prime`'s transform butterfly."""
+    r = ctypes.c_double()
+    _check(lib().nnh_butterfly_peak(prime, iters, ctypes.byref(r)))
+    return r.value
 
 
 def kernel_times():

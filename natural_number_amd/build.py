@@ -17,7 +17,7 @@ ARCH = os.environ.get("NNH_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function"]
 
-SOURCES = ["nnh_mid.hip", "nnh_mid_lazy.hip", "nnh_top.hip", "nnh_top_lazy.hip", "nnh_crt.hip", "nnh_mlp.hip",
+SOURCES = ["nnh_mid.hip", "nnh_mid_lazy.hip", "nnh_top.hip", "nnh_top_lazy.hip", "nnh_crt.hip", "nnh_mlp.hip", "nnh_peak.hip",
            "nnh_nat.hip", "nnh_api.cpp", "nnh_natural.cpp"]
 
 

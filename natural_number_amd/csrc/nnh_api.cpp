@@ -980,6 +980,14 @@ int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an,
     return NNH_OK;
 }
 
+int nnh_butterfly_peak(int prime, uint64_t iters, double* butterflies_per_s) {
+    if (prime < 0 || prime > 2 || iters < 1 || iters > (1u << 20) || !butterflies_per_s)
+        return fail(NNH_ERR_ARG, "bad butterfly-peak arguments");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    return butterfly_peak(prime, (u32)iters, butterflies_per_s);
+}
+
 int nnh_time_algo(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn, int algo,
                   int param, int count, int reps, float* ms_per_product) {
     if (!d_dst || !d_a || !d_b || !an || !bn || count < 1 || reps < 1 || !ms_per_product)

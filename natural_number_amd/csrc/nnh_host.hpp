@@ -42,4 +42,7 @@ struct IReq {
 // Plan, enqueue, synchronise and release one batch.
 int dev_batch(const IReq* reqs, u64 n, hipStream_t st);
 
+// Register-resident butterfly rate of prime pr (nnh_peak.hip).
+int butterfly_peak(int pr, u32 iters, double* rate);
+
 }  // namespace nnh

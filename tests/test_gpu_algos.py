@@ -67,3 +67,13 @@ def test_mlp_gradients_vs_finite_differences(nnh):
     l1 = model.train(X, T, M, steps=300, lr=3e-3)
     assert l1 < loss
     model.close()
+
+
+def test_butterfly_peak_rates(nnh):
+    """The butterfly-issue ceiling runs for every prime and gives a plausible
+    rate (between 10^10 and 10^14 butterflies per second); bad arguments fail."""
+    for pr in range(3):
+        r = nnh.butterfly_peak(pr, 32)
+        assert 1e10 < r < 1e14, (pr, r)
+    with pytest.raises(nnh.NNHError):
+        nnh.butterfly_peak(3, 32)
