@@ -128,8 +128,23 @@ __device__ __forceinline__ T sel3(const T (&a)[3], u32 i) {
 // here the scale alpha^2 = 2^64 * 2^-(s-1) absorbs both the base-multiply
 // Montgomery factor and the inverse transform's 2^(s-1).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ u64 load_limb(const Prime& P, const u64* __restrict__ a, u64 an, u64 x, u64 alpha_m) {
-    return x < an ? mont(P, a[x], alpha_m) : 0;
+// Limbs of a thread's first N elements (element j at index xs(j)) as
+// residues: all loads first, then the reductions, so that the loads issue
+// back to back.  The index is clamped into the operand and the limb masked
+// to zero beyond it (the reduction of 0 is p, which is 0 mod p).  an >= 1.
+template <int N, class XS>
+__device__ __forceinline__ void load_limbs(const Prime& P, u64 (&v)[16], const u64* __restrict__ a, u64 an, u64 alpha_m,
+                                           XS&& xs) {
+    u64 w[N];
+    CFor<0, N>::run([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const u64 x = xs(j);
+        w[j] = a[x < an ? x : an - 1];
+    });
+    CFor<0, N>::run([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        v[j] = mont(P, xs(j) < an ? w[j] : 0, alpha_m);
+    });
 }
 
 struct TopArgs {
@@ -170,17 +185,17 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
     const bool top_copy = FROM_LIMBS && !INV && blk == 0 && (d << (NLEV - 1)) >= A.an;
     {
         const u32 yt = lay_t(t, LO_IO);
-        CFor<0, 16>::run([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            constexpr u32 jy = (u32)j << LO_IO;
-            const u32 y = yt | jy;
-            const u64 x = xbase + (y & QM) + (u64)(y >> Q) * d;
-            if constexpr (FROM_LIMBS) {
-                if (j < 8 || !top_copy) v[j] = load_limb(P, A.limbs, A.an, x, alpha);
-            } else {
-                v[j] = data[x];
-            }
-        });
+        auto xs = [&](int j) {
+            const u32 y = yt | ((u32)j << LO_IO);
+            return xbase + (y & QM) + (u64)(y >> Q) * d;
+        };
+        if constexpr (FROM_LIMBS) {
+            if (A.an == 0) CFor<0, 16>::run([&](auto jc) { v[decltype(jc)::value] = 0; });
+            else if (top_copy) load_limbs<8>(P, v, A.limbs, A.an, alpha, xs);
+            else load_limbs<16>(P, v, A.limbs, A.an, alpha, xs);
+        } else {
+            CFor<0, 16>::run([&](auto jc) { v[decltype(jc)::value] = data[xs(decltype(jc)::value)]; });
+        }
     }
     if constexpr (!INV) {
         static_assert(LO_IO == CB - 4, "the top level's partner is register j + 8");
@@ -249,12 +264,13 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
     auto twi_ = [&]() -> const u64* { if constexpr (SEL) return twi_s; else return A.twi[pr]; };
     auto alpha_ = [&]() -> u64 { if constexpr (SEL) return alpha_s; else return A.alpha_m[pr]; };
     u64 a[16], b[16];
-    CFor<0, 16>::run([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        const u64 x = base + (yt | ((u32)j << LO_IO));
-        if constexpr (FROM_LIMBS) a[j] = load_limb(P, A.la, A.an, x, alpha_());
-        else a[j] = fa_()[x];
-    });
+    auto xs = [&](int j) { return base + (yt | ((u32)j << LO_IO)); };
+    if constexpr (FROM_LIMBS) {
+        if (A.an == 0) CFor<0, 16>::run([&](auto jc) { a[decltype(jc)::value] = 0; });
+        else load_limbs<16>(P, a, A.la, A.an, alpha_(), xs);
+    } else {
+        CFor<0, 16>::run([&](auto jc) { a[decltype(jc)::value] = fa_()[xs(decltype(jc)::value)]; });
+    }
     if constexpr (FWD_ONLY) {
         fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0, LAZY>(P, a, lds, t, chunk, twf_());
         if constexpr (LO_F != LO_IO) xchg<CB, LO_F, LO_IO>(a, lds, t);
@@ -271,12 +287,9 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
             xchg<CB, LO_IO, 0>(a, lds, t);
         }
         if (!SQR && !sqr_rt) {
-            CFor<0, 16>::run([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                const u64 x = base + (yt | ((u32)j << LO_IO));
-                if (A.lb) b[j] = load_limb(P, A.lb, A.bn, x, alpha_());
-                else b[j] = fb_()[x];
-            });
+            if (A.lb && A.bn) load_limbs<16>(P, b, A.lb, A.bn, alpha_(), xs);
+            else if (A.lb) CFor<0, 16>::run([&](auto jc) { b[decltype(jc)::value] = 0; });
+            else CFor<0, 16>::run([&](auto jc) { b[decltype(jc)::value] = fb_()[xs(decltype(jc)::value)]; });
             if (!A.b_done) {
                 fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0, LAZY>(P, b, lds, t, chunk, twf_());
                 if constexpr (LO_F != 0) xchg<CB, LO_F, 0>(b, lds, t);
