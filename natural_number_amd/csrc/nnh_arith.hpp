@@ -61,10 +61,72 @@ __host__ __device__ __forceinline__ u64 redc(const Prime& P, u64 lo, u64 hi) {
     return hi + P.p - N;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Device form of mont(): the same reduction with the 64x64 product's
+// column sums carried through SGPR carry flags instead of zero-extended
+// 64-bit addends (no register moves), and floor((l k + (mh >> 24)) / 2^8)
+// taken from the one multiply G = l * (k << 24) = (byte << 24) + 2^32 *
+// floor(l k / 2^8) plus the carry of (byte << 24) + (mh & 0xff000000).
+// Result (T_hi + p) - mh*k*2^24 - q in (0, 2p), as redc().  Every SGPR carry
+// is read at least two instructions after the VALU instruction that wrote
+// it, counting only instructions inside these blocks (gfx950 requires two
+// wait states between a VALU SGPR write and a VALU read of it), so the
+// order in which the compiler places the blocks cannot break it.
+// Requires T = a*b < p 2^64 and b < 2^63 (so y1*w1 + p does not wrap).
+__device__ __forceinline__ u64 mont_dev(const Prime& P, u64 y, u64 w) {
+    u64 A, B, C, D, cM, d1;
+    asm("v_mad_u64_u32 %[A], %[d1], %[y0], %[w0], 0\n\t"
+        "v_mad_u64_u32 %[B], %[d1], %[y1], %[w0], 0\n\t"
+        "v_mad_u64_u32 %[C], %[cM], %[y0], %[w1], %[B]\n\t"
+        "v_mad_u64_u32 %[D], %[d1], %[y1], %[w1], %[pp]"
+        : [A] "=&v"(A), [B] "=&v"(B), [C] "=&v"(C), [D] "=&v"(D), [cM] "=&s"(cM), [d1] "=&s"(d1)
+        : [y0] "v"(lo32(y)), [y1] "v"(hi32(y)), [w0] "v"(lo32(w)), [w1] "v"(hi32(w)), [pp] "s"(P.p));
+    // A = y0 w0, C = y0 w1 + y1 w0 (carry cM = 2^64), D = y1 w1 + p
+    u32 L1;
+    u64 c2, G, dG;
+    asm("v_add_co_u32_e64 %[L1], %[c2], %[h0], %[c0]\n\t"
+        "v_mad_u64_u32 %[G], %[dG], %[l0], %[k24], 0"
+        : [L1] "=&v"(L1), [c2] "=&s"(c2), [G] "=&v"(G), [dG] "=&s"(dG)
+        : [h0] "v"(hi32(A)), [c0] "v"(lo32(C)), [l0] "v"(lo32(A)), [k24] "s"(P.k24));
+    // L1 = T bits 32-63; E = T_hi + p; mh = L1 - byte << 24; cq = carry of the byte sum
+    u32 E0, E1, mh, shh, dq;
+    u64 c3, cq, d2;
+    asm("s_nop 0\n\t"
+        "v_addc_co_u32_e64 %[E0], %[c3], %[d0], %[c1], %[c2]\n\t"
+        "v_sub_u32 %[mh], %[L1], %[glo]\n\t"
+        "v_and_b32 %[shh], 0xff000000, %[mh]\n\t"
+        "v_addc_co_u32_e64 %[E1], %[d2], %[dd1], 0, %[c3]\n\t"
+        "v_add_co_u32_e64 %[dq], %[cq], %[glo], %[shh]\n\t"
+        "v_addc_co_u32_e64 %[E1], %[d2], %[E1], 0, %[cM]"
+        : [E0] "=&v"(E0), [E1] "=&v"(E1), [mh] "=&v"(mh), [shh] "=&v"(shh), [dq] "=&v"(dq), [c3] "=&s"(c3),
+          [cq] "=&s"(cq), [d2] "=&s"(d2)
+        : [d0] "v"(lo32(D)), [c1] "v"(hi32(C)), [c2] "s"(c2), [L1] "v"(L1), [glo] "v"(lo32(G)), [dd1] "v"(hi32(D)),
+          [cM] "s"(cM));
+    // F = E + mh (2^32 - k 2^24) = E - mh k 2^24 + mh 2^32
+    u64 F, dF;
+    asm("v_mad_u64_u32 %[F], %[dF], %[mh], %[nk], %[E]"
+        : [F] "=&v"(F), [dF] "=&s"(dF)
+        : [mh] "v"(mh), [nk] "s"(0u - P.k24), [E] "v"(mk64(E0, E1)));
+    // t = F - (q0 + cq) - mh 2^32
+    u32 T0, T1;
+    u64 b1, b2;
+    asm("v_subb_co_u32_e64 %[T0], %[b1], %[f0], %[q0], %[cq]\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %[T1], %[b2], %[f1], %[mh], %[b1]"
+        : [T0] "=&v"(T0), [T1] "=&v"(T1), [b1] "=&s"(b1), [b2] "=&s"(b2)
+        : [f0] "v"(lo32(F)), [f1] "v"(hi32(F)), [q0] "v"(hi32(G)), [mh] "v"(mh), [cq] "s"(cq));
+    return mk64(T0, T1);
+}
+#endif
+
 __host__ __device__ __forceinline__ u64 mont(const Prime& P, u64 a, u64 b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return mont_dev(P, a, b);
+#else
     u64 lo, hi;
     mul_wide(a, b, lo, hi);
     return redc(P, lo, hi);
+#endif
 }
 
 // Sum of two products, reduced: (a*b + c*d) * 2^-64 mod p in [0, 2p);

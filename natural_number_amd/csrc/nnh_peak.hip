@@ -63,6 +63,7 @@ int butterfly_peak(int pr, u32 iters, double* rate) {
         chk(hipEventCreate(&e1))) {
         hipLaunchKernelGGL(k_peak_fill, dim3(1024), dim3(256), 0, st, d, n, p);
         hipLaunchKernelGGL(k_peak_fill, dim3(4), dim3(256), 0, st, tw, (u64)1024, p);
+        chk(hipGetLastError());
         float best = 1e30f;
         for (int rep = 0; rep < 4 && rc == NNH_OK; rep++) {
             chk(hipEventRecord(e0, st));
@@ -70,6 +71,7 @@ int butterfly_peak(int pr, u32 iters, double* rate) {
                 hipLaunchKernelGGL(k_bfly_peak<true>, dim3(PEAK_BLOCKS), dim3(PEAK_THREADS), 0, st, d, tw, (u32)pr, iters);
             else
                 hipLaunchKernelGGL(k_bfly_peak<false>, dim3(PEAK_BLOCKS), dim3(PEAK_THREADS), 0, st, d, tw, (u32)pr, iters);
+            chk(hipGetLastError());
             chk(hipEventRecord(e1, st));
             chk(hipEventSynchronize(e1));
             float ms = 0;
