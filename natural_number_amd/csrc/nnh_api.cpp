@@ -739,6 +739,12 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
         if (it.s == 0) B->n_school++;
         B->jobs.push_back(J);
     }
+    // the batched kernels launch one work-group per (job, tile) item of a bucket
+    for (const Bucket& bk : B->buckets)
+        if ((u64)bk.count * ((1ull << bk.s) >> MIN_MID_BITS) >= (1ull << 31)) {
+            delete B;
+            return fail(NNH_ERR_ARG, "batch bucket too large");
+        }
     // forward-only entries: transform each shared B once into its owner's fb
     B->n_req_jobs = (u32)B->jobs.size();
     for (Bucket& bk : B->buckets) {

@@ -14,38 +14,42 @@ __global__ __launch_bounds__(1 << (CB - 4)) void k_mid(MidArgs A, u32 pr0) {
     mid_body<CB, FROM_LIMBS, SQR, FWD_ONLY, false, LAZY>(A, pr, blockIdx.x, lds);
 }
 
-// FWD_ONLY: the jobs are the forward-only entries of shared B transforms
-// (fa = the owner's fb, a = its B limbs); the finished transform is stored.
-// Otherwise a job with share != 0 reads B's finished transform from fb.
+// Batched: one work-group per (job, chunk) item (blockIdx.x), prime
+// pr0 + blockIdx.y, as in the batched top pass (one item per work-group
+// keeps the single-product register budget; per-prime pointers are resolved
+// before the body).  FWD_ONLY: the jobs are the forward-only entries of
+// shared B transforms (fa = the owner's fb, a = its B limbs); the finished
+// transform is stored.  Otherwise a job with share != 0 reads B's finished
+// transform from fb.
 template <int CB, bool FROM_LIMBS, bool FWD_ONLY, bool LAZY>
 __global__ __launch_bounds__(1 << (CB - 4)) void k_mid_batch(const BJob* __restrict__ jobs, u32 njobs,
-                                                            MidBatchParams P, u32 pr0, u32 npr) {
+                                                            MidBatchParams P, u32 pr0) {
     __shared__ u64 lds[(1 << CB) + (1 << (CB - 4))];
     const u64 cpj = P.L >> CB;
-    const u64 items = (u64)njobs * cpj * npr;
-    for (u64 it = blockIdx.x; it < items; it += gridDim.x) {
-        const u32 pr = pr0 + (u32)(it % npr);
-        const u64 r = it / npr;
-        const u64 chunk = r % cpj;
-        const BJob& J = jobs[r / cpj];
-        MidArgs A;
-        for (int i = 0; i < 3; i++) {
-            A.fa[i] = J.fa + (u64)i * P.L;
-            A.fb[i] = J.fb + (u64)i * P.L;
-            A.twf[i] = P.twf[i];
-            A.twi[i] = P.twi[i];
-            A.alpha_m[i] = P.alpha_m[i];
-        }
-        A.la = J.a;
-        A.lb = FROM_LIMBS && !J.share ? J.b : nullptr;
-        A.an = J.an;
-        A.bn = J.bn;
-        A.a_done = 0;
-        A.b_done = J.share != 0;
-        bool sq = J.sqr != 0;
-        mid_body<CB, FROM_LIMBS, false, FWD_ONLY, true, LAZY>(A, pr, chunk, lds, sq);
-        __syncthreads();
+    const u32 pr = pr0 + blockIdx.y;
+    const u64 chunk = blockIdx.x % cpj;
+    const BJob& J = jobs[blockIdx.x / cpj];
+    MidArgs A;
+    u64* const fa = J.fa + (u64)pr * P.L;
+    const u64* const fb = J.fb + (u64)pr * P.L;
+    const u64* const twf = sel3(P.twf, pr);
+    const u64* const twi = sel3(P.twi, pr);
+    const u64 alpha = sel3(P.alpha_m, pr);
+    for (int i = 0; i < 3; i++) {
+        A.fa[i] = fa;
+        A.fb[i] = fb;
+        A.twf[i] = twf;
+        A.twi[i] = twi;
+        A.alpha_m[i] = alpha;
     }
+    A.la = J.a;
+    A.lb = FROM_LIMBS && !J.share ? J.b : nullptr;
+    A.an = J.an;
+    A.bn = J.bn;
+    A.a_done = 0;
+    A.b_done = J.share != 0;
+    bool sq = J.sqr != 0;
+    mid_body<CB, FROM_LIMBS, false, FWD_ONLY, true, LAZY>(A, pr, chunk, lds, sq, 0);
 }
 
 template <int CB, bool LAZY>
@@ -90,10 +94,8 @@ template <int CB, bool LAZY>
 static void mid_batch_dispatch(const BJob* jobs, u32 njobs, const MidBatchParams& P, bool from_limbs, bool fwd_only,
                                hipStream_t st) {
     const u32 pr0 = LAZY ? 0 : 2, npr = LAZY ? 2 : 1;
-    u64 items = (u64)njobs * (P.L >> CB) * npr;
-    dim3 grid(persist_grid(items, 4096)), block(1 << (CB - 4));
-#define NNH_MB(fl, fo) \
-    hipLaunchKernelGGL((k_mid_batch<CB, fl, fo, LAZY>), grid, block, 0, st, jobs, njobs, P, pr0, npr)
+    dim3 grid((unsigned)((u64)njobs * (P.L >> CB)), npr), block(1 << (CB - 4));
+#define NNH_MB(fl, fo) hipLaunchKernelGGL((k_mid_batch<CB, fl, fo, LAZY>), grid, block, 0, st, jobs, njobs, P, pr0)
     if (fwd_only) {
         if (from_limbs) NNH_MB(true, true);
         else NNH_MB(false, true);

@@ -45,16 +45,23 @@ __global__ __launch_bounds__(64) void k_forward(const float* __restrict__ P, Dim
                                                 int n, float* __restrict__ y, float* __restrict__ h1s,
                                                 float* __restrict__ h2s) {
     __shared__ float sp[H * MAX_IN + H + H * H + H + MAX_OUT * H + MAX_OUT];
+    // per-thread activations live in LDS rows (indexed by loop counters that
+    // a register array could only serve through scratch memory)
+    __shared__ float sh1[64][H + 1], sh2[64][H + 1];
     for (int i = threadIdx.x; i < d.n; i += blockDim.x) sp[i] = P[i];
     __syncthreads();
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     float xi[MAX_IN];
-    for (int k = 0; k < d.ni; k++) xi[k] = x[(size_t)s * d.ni + k];
-    float h1[H], h2[H];
+#pragma unroll
+    for (int k = 0; k < MAX_IN; k++) xi[k] = k < d.ni ? x[(size_t)s * d.ni + k] : 0.f;
+    float* h1 = sh1[threadIdx.x];
+    float* h2 = sh2[threadIdx.x];
     for (int j = 0; j < H; j++) {
         float a = sp[d.b1 + j];
-        for (int k = 0; k < d.ni; k++) a = fmaf(sp[d.w1 + j * d.ni + k], xi[k], a);
+#pragma unroll
+        for (int k = 0; k < MAX_IN; k++)
+            if (k < d.ni) a = fmaf(sp[d.w1 + j * d.ni + k], xi[k], a);
         h1[j] = a > 0.f ? a : 0.f;
     }
     for (int j = 0; j < H; j++) {

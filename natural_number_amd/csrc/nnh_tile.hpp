@@ -243,8 +243,12 @@ struct MidArgs {
 // SEL: pick the per-prime pointers with selects (an argument struct built
 // in registers, batched kernels) instead of indexing the kernel-argument
 // struct directly (single-product kernel, where that keeps fewer registers).
+// slot: the entry of A's per-prime arrays to use when SEL (pr if omitted; a
+// caller that stores one prime's pointers in every entry passes 0, which
+// keeps the selection out of a run-time-indexed stack array).
 template <int CB, bool FROM_LIMBS, bool SQR, bool FWD_ONLY, bool SEL, bool LAZY>
-__device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u64* lds, bool sqr_rt = false) {
+__device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u64* lds, bool sqr_rt = false,
+                                         u32 slot = ~0u) {
     const Prime P = make_prime(PRIME_K[0] * (pr == 0) + PRIME_K[1] * (pr == 1) + PRIME_K[2] * (pr == 2));
     constexpr int LO_IO = CB - 4;
     constexpr int LO_F = fwd_window(1, CB - 1, LO_IO, 0);    // window after the forward levels
@@ -253,11 +257,12 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
     const u32 yt = lay_t(t, LO_IO);
     // SEL: selected once; otherwise indexed at each use, which lets the
     // compiler rematerialise the kernel-argument loads
-    u64* const fa_s = SEL ? sel3(A.fa, pr) : nullptr;
-    const u64* const fb_s = SEL ? sel3(A.fb, pr) : nullptr;
-    const u64* const twf_s = SEL ? sel3(A.twf, pr) : nullptr;
-    const u64* const twi_s = SEL ? sel3(A.twi, pr) : nullptr;
-    const u64 alpha_s = SEL ? sel3(A.alpha_m, pr) : 0;
+    const u32 sl = slot == ~0u ? pr : slot;
+    u64* const fa_s = SEL ? sel3(A.fa, sl) : nullptr;
+    const u64* const fb_s = SEL ? sel3(A.fb, sl) : nullptr;
+    const u64* const twf_s = SEL ? sel3(A.twf, sl) : nullptr;
+    const u64* const twi_s = SEL ? sel3(A.twi, sl) : nullptr;
+    const u64 alpha_s = SEL ? sel3(A.alpha_m, sl) : 0;
     auto fa_ = [&]() -> u64* { if constexpr (SEL) return fa_s; else return A.fa[pr]; };
     auto fb_ = [&]() -> const u64* { if constexpr (SEL) return fb_s; else return A.fb[pr]; };
     auto twf_ = [&]() -> const u64* { if constexpr (SEL) return twf_s; else return A.twf[pr]; };

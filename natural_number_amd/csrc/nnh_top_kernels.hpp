@@ -14,34 +14,35 @@ __global__ __launch_bounds__(256) void k_top_pass(TopArgs A, u32 pr0) {
     top_pass_body<INV, FROM_LIMBS, NLEV, LAZY>(A, pr0 + blockIdx.y, blockIdx.x, lds);
 }
 
-// Batched: persistent grid-stride loop over (job, tile, prime) items of a
-// bucket of equal transform length, primes pr0 .. pr0+npr-1.
+// Batched: one work-group per (job, tile) item of a bucket of equal
+// transform length (blockIdx.x), prime pr0 + blockIdx.y.  One item per
+// work-group keeps the register budget of the single-product kernel (a
+// grid-stride loop over items kept its loop state live and cost ~60 VGPRs,
+// one wave per SIMD); the per-prime pointers are resolved before the body,
+// so nothing is indexed by the prime at run time.
 template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY>
 __global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__ jobs, u32 njobs, int operand,
-                                                        TopBatchParams P, u32 pr0, u32 npr) {
+                                                        TopBatchParams P, u32 pr0) {
     __shared__ u64 lds[4096 + 256];
     const u64 tpj = P.L >> 12;
-    const u64 items = (u64)njobs * tpj * npr;
-    for (u64 it = blockIdx.x; it < items; it += gridDim.x) {
-        const u32 pr = pr0 + (u32)(it % npr);
-        const u64 r = it / npr;
-        const u64 tile = r % tpj;
-        const BJob& J = jobs[r / tpj];
-        if (operand && (J.sqr || J.share == 2)) continue;  // uniform per work-group
-        TopArgs A;
-        u64* base = operand ? J.fb : J.fa;
-        for (int i = 0; i < 3; i++) {
-            A.data[i] = base + (u64)i * P.L;
-            A.tw[i] = P.tw[i];
-            A.alpha_m[i] = P.alpha_m[i];
-        }
-        A.limbs = operand ? J.b : J.a;
-        A.an = operand ? J.bn : J.an;
-        A.g_lo = P.g_lo;
-        A.nlev = P.nlev;
-        top_pass_body<INV, FROM_LIMBS, NLEV, LAZY>(A, pr, tile, lds);
-        __syncthreads();
+    const u32 pr = pr0 + blockIdx.y;
+    const u64 tile = blockIdx.x % tpj;
+    const BJob& J = jobs[blockIdx.x / tpj];
+    if (operand && (J.sqr || J.share == 2)) return;  // uniform per work-group
+    TopArgs A;
+    u64* const data = (operand ? J.fb : J.fa) + (u64)pr * P.L;
+    const u64* const tw = sel3(P.tw, pr);
+    const u64 alpha = sel3(P.alpha_m, pr);
+    for (int i = 0; i < 3; i++) {
+        A.data[i] = data;
+        A.tw[i] = tw;
+        A.alpha_m[i] = alpha;
     }
+    A.limbs = operand ? J.b : J.a;
+    A.an = operand ? J.bn : J.an;
+    A.g_lo = P.g_lo;
+    A.nlev = P.nlev;
+    top_pass_body<INV, FROM_LIMBS, NLEV, LAZY>(A, pr, tile, lds);
 }
 
 // Host dispatch on the pass depth (1..8 levels), known per launch.
@@ -56,17 +57,17 @@ static void top_single(const TopArgs& A, u32 nlev, dim3 grid, u32 pr0, hipStream
     }
 }
 template <bool INV, bool FL, bool LAZY>
-static void top_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, dim3 grid, u32 pr0, u32 npr,
+static void top_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, dim3 grid, u32 pr0,
                       hipStream_t st) {
     switch (P.nlev) {
 #define NNH_TP(n)                                                                                     \
     case n:                                                                                           \
         hipLaunchKernelGGL((k_top_pass_batch<INV, FL, n, LAZY>), grid, dim3(256), 0, st, jobs, njobs, \
-                           operand, P, pr0, npr);                                                     \
+                           operand, P, pr0);                                                          \
         break;
         NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
         default: hipLaunchKernelGGL((k_top_pass_batch<INV, FL, 8, LAZY>), grid, dim3(256), 0, st, jobs, njobs,
-                                    operand, P, pr0, npr);
+                                    operand, P, pr0);
             break;
 #undef NNH_TP
     }
@@ -95,11 +96,10 @@ template <bool LAZY>
 static void top_pass_batch_primes(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
                                   bool from_limbs, hipStream_t st) {
     const u32 pr0 = LAZY ? 0 : 2, npr = LAZY ? 2 : 1;
-    u64 items = (u64)njobs * (P.L >> 12) * npr;
-    dim3 grid(persist_grid(items, 4096));
-    if (inverse) top_batch<true, false, LAZY>(jobs, njobs, operand, P, grid, pr0, npr, st);
-    else if (from_limbs) top_batch<false, true, LAZY>(jobs, njobs, operand, P, grid, pr0, npr, st);
-    else top_batch<false, false, LAZY>(jobs, njobs, operand, P, grid, pr0, npr, st);
+    dim3 grid((unsigned)((u64)njobs * (P.L >> 12)), npr);
+    if (inverse) top_batch<true, false, LAZY>(jobs, njobs, operand, P, grid, pr0, st);
+    else if (from_limbs) top_batch<false, true, LAZY>(jobs, njobs, operand, P, grid, pr0, st);
+    else top_batch<false, false, LAZY>(jobs, njobs, operand, P, grid, pr0, st);
 }
 
 }  // namespace nnh
