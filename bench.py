@@ -49,7 +49,26 @@ def cpu_baseline(a, b):
     return out, time.perf_counter() - t
 
 
-def batch_mix(nn, stream, reps=5):
+def batch_ideal_ms(sizes, r_lazy, r_strict):
+    """Butterfly-issue ideal of the batch: every transform job at the ceiling
+    rates (schoolbook jobs below 384 words not counted).  A job at scale
+    s = bit_length(an + bn - 2) runs s - 1 butterfly levels of 2^(s-1) per
+    transform and prime (forward A, forward B, inverse); with top passes
+    (s > 12) the load pass's top level is a copy for an operand of at most
+    2^(s-1) words."""
+    per = 2.0 / r_lazy + 1.0 / r_strict
+    t = 0.0
+    for an, bn in sizes:
+        if min(an, bn) < 384:
+            continue
+        s = (an + bn - 2).bit_length()
+        half = 1 << (s - 1)
+        lv = 3 * (s - 1) - sum(1 for n in (an, bn) if s > 12 and n <= half)
+        t += lv * half * per
+    return t * 1e3
+
+
+def batch_mix(nn, stream, valu, reps=5):
     """Config 3: 4096 uneven products; batched (bucketed persistent kernels,
     one hipGraph) against one launch sequence per product on the same stream."""
     from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
@@ -95,6 +114,7 @@ def batch_mix(nn, stream, reps=5):
     for x in (inp, out_b, out_s):
         x.free()
     n = len(sizes)
+    ideal = batch_ideal_ms(sizes, valu["peak_lazy_bfly_per_s"], valu["peak_strict_bfly_per_s"])
     return {
         "workload": "4096 products, operand bits log-uniform in [2^12, 2^24], half balanced, half up to 64:1 "
                     "(generator seed 0x4E4E2E1972800003, splitmix64 operands)",
@@ -105,6 +125,7 @@ def batch_mix(nn, stream, reps=5):
         "per_product_launch_products_per_s": n / (ts * 1e-3),
         "speedup": ts / tb,
         "bit_exact_batched_vs_single": exact,
+        "valu_ceiling": {"ideal_ms": round(ideal, 3), "frac": round(ideal / tb, 3)},
     }
 
 
@@ -377,7 +398,7 @@ def main():
 
     valu = valu_ceiling(nn, kernels, args.steps)
 
-    batch = None if args.no_batch else batch_mix(nn, stream)
+    batch = None if args.no_batch else batch_mix(nn, stream, valu)
     e_res = None if args.no_e else e_workload(nn, not args.no_cpu)
     drv = None if args.no_drivers else drivers_workload(nn, not args.no_cpu)
 
