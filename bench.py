@@ -69,7 +69,7 @@ def batch_ideal_ms(sizes, r_lazy, r_strict):
 
 
 def batch_mix(nn, stream, valu, reps=5):
-    """Config 3: 4096 uneven products; batched (bucketed persistent kernels,
+    """Config 3: 4096 uneven products; batched (bucketed kernels,
     one hipGraph) against one launch sequence per product on the same stream."""
     from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
 

@@ -127,8 +127,11 @@ int nnh_mul_transformed(uint64_t* dst, const uint64_t* fa, uint64_t an, const ui
  * pair at a time; needed for binary-splitting workloads such as calc_e,
  * natural.h:841-857).  Requests carry DEVICE pointers; dst holds an+bn words.
  * A request with a == b and an == bn is a squaring.  Products are bucketed
- * by transform length; each transform stage is one persistent launch per
- * bucket and one CRT/carry sequence serves the whole batch. */
+ * by transform length; each transform stage is one launch per
+ * bucket and one CRT/carry sequence serves the whole batch.  Requests that
+ * repeat a B operand (same pointer and size, same transform length) share
+ * one transform of it, as the reference's mul_save/mul_load reuse does
+ * (natural.h:452-527); the products are the same. */
 typedef struct nnh_req {
     uint64_t* dst;
     const uint64_t* a;
