@@ -37,10 +37,14 @@ __host__ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi
 struct Prime {
     u64 p;
     u64 p2;   // 2p
+    u64 np;   // -p mod 2^64
+    u64 n2p;  // -2p mod 2^64
     u32 k;
     u32 k24;  // k << 24
 };
-__host__ __device__ constexpr Prime make_prime(u32 k) { return Prime{prime_p(k), prime_p(k) << 1, k, k << 24}; }
+__host__ __device__ constexpr Prime make_prime(u32 k) {
+    return Prime{prime_p(k), prime_p(k) << 1, 0 - prime_p(k), 0 - (prime_p(k) << 1), k, k << 24};
+}
 // 4p < 2^64 holds for p0 and p1 (k = 27, 58) but not p2 (k = 87): only the
 // first two primes use the lazy [0, 4p) butterflies below.
 __host__ __device__ constexpr bool lazy_prime(int i) { return i < 2; }
@@ -72,23 +76,36 @@ __host__ __device__ __forceinline__ u64 redc(const Prime& P, u64 lo, u64 hi) {
 // it, counting only instructions inside these blocks (gfx950 requires two
 // wait states between a VALU SGPR write and a VALU read of it), so the
 // order in which the compiler places the blocks cannot break it.
-// Requires T = a*b < p 2^64 and b < 2^63 (so y1*w1 + p does not wrap).
-__device__ __forceinline__ u64 mont_dev(const Prime& P, u64 y, u64 w) {
+// mont_dev_add adds a 64-bit addend c in place of p: the result is
+// T_hi + c - mh*k*2^24 - q = c - p + (T 2^-64 mod p, in (0, 2p)).
+// Requires T = a*b < p 2^64, y1*w1 + c < 2^64 and T_hi + c < 2^64 (for
+// c = p: b < 2^63).
+template <bool VADD>
+__device__ __forceinline__ u64 mont_dev_add(const Prime& P, u64 y, u64 w, u64 c) {
     u64 A, B, C, D, cM, d1;
-    asm("v_mad_u64_u32 %[A], %[d1], %[y0], %[w0], 0\n\t"
-        "v_mad_u64_u32 %[B], %[d1], %[y1], %[w0], 0\n\t"
-        "v_mad_u64_u32 %[C], %[cM], %[y0], %[w1], %[B]\n\t"
-        "v_mad_u64_u32 %[D], %[d1], %[y1], %[w1], %[pp]"
-        : [A] "=&v"(A), [B] "=&v"(B), [C] "=&v"(C), [D] "=&v"(D), [cM] "=&s"(cM), [d1] "=&s"(d1)
-        : [y0] "v"(lo32(y)), [y1] "v"(hi32(y)), [w0] "v"(lo32(w)), [w1] "v"(hi32(w)), [pp] "s"(P.p));
-    // A = y0 w0, C = y0 w1 + y1 w0 (carry cM = 2^64), D = y1 w1 + p
+    if constexpr (VADD) {
+        asm("v_mad_u64_u32 %[A], %[d1], %[y0], %[w0], 0\n\t"
+            "v_mad_u64_u32 %[B], %[d1], %[y1], %[w0], 0\n\t"
+            "v_mad_u64_u32 %[C], %[cM], %[y0], %[w1], %[B]\n\t"
+            "v_mad_u64_u32 %[D], %[d1], %[y1], %[w1], %[pp]"
+            : [A] "=&v"(A), [B] "=&v"(B), [C] "=&v"(C), [D] "=&v"(D), [cM] "=&s"(cM), [d1] "=&s"(d1)
+            : [y0] "v"(lo32(y)), [y1] "v"(hi32(y)), [w0] "v"(lo32(w)), [w1] "v"(hi32(w)), [pp] "v"(c));
+    } else {
+        asm("v_mad_u64_u32 %[A], %[d1], %[y0], %[w0], 0\n\t"
+            "v_mad_u64_u32 %[B], %[d1], %[y1], %[w0], 0\n\t"
+            "v_mad_u64_u32 %[C], %[cM], %[y0], %[w1], %[B]\n\t"
+            "v_mad_u64_u32 %[D], %[d1], %[y1], %[w1], %[pp]"
+            : [A] "=&v"(A), [B] "=&v"(B), [C] "=&v"(C), [D] "=&v"(D), [cM] "=&s"(cM), [d1] "=&s"(d1)
+            : [y0] "v"(lo32(y)), [y1] "v"(hi32(y)), [w0] "v"(lo32(w)), [w1] "v"(hi32(w)), [pp] "s"(c));
+    }
+    // A = y0 w0, C = y0 w1 + y1 w0 (carry cM = 2^64), D = y1 w1 + c
     u32 L1;
     u64 c2, G, dG;
     asm("v_add_co_u32_e64 %[L1], %[c2], %[h0], %[c0]\n\t"
         "v_mad_u64_u32 %[G], %[dG], %[l0], %[k24], 0"
         : [L1] "=&v"(L1), [c2] "=&s"(c2), [G] "=&v"(G), [dG] "=&s"(dG)
         : [h0] "v"(hi32(A)), [c0] "v"(lo32(C)), [l0] "v"(lo32(A)), [k24] "s"(P.k24));
-    // L1 = T bits 32-63; E = T_hi + p; mh = L1 - byte << 24; cq = carry of the byte sum
+    // L1 = T bits 32-63; E = T_hi + c; mh = L1 - byte << 24; cq = carry of the byte sum
     u32 E0, E1, mh, shh, dq;
     u64 c3, cq, d2;
     asm("s_nop 0\n\t"
@@ -108,6 +125,60 @@ __device__ __forceinline__ u64 mont_dev(const Prime& P, u64 y, u64 w) {
         : [F] "=&v"(F), [dF] "=&s"(dF)
         : [mh] "v"(mh), [nk] "s"(0u - P.k24), [E] "v"(mk64(E0, E1)));
     // t = F - (q0 + cq) - mh 2^32
+    u32 T0, T1;
+    u64 b1, b2;
+    asm("v_subb_co_u32_e64 %[T0], %[b1], %[f0], %[q0], %[cq]\n\t"
+        "s_nop 1\n\t"
+        "v_subb_co_u32_e64 %[T1], %[b2], %[f1], %[mh], %[b1]"
+        : [T0] "=&v"(T0), [T1] "=&v"(T1), [b1] "=&s"(b1), [b2] "=&s"(b2)
+        : [f0] "v"(lo32(F)), [f1] "v"(hi32(F)), [q0] "v"(hi32(G)), [mh] "v"(mh), [cq] "s"(cq));
+    return mk64(T0, T1);
+}
+__device__ __forceinline__ u64 mont_dev(const Prime& P, u64 y, u64 w) { return mont_dev_add<false>(P, y, w, P.p); }
+
+// (a*b + c*d) * 2^-64 mod p in (0, 2p) for p0 and p1 and canonical inputs
+// (below p, so every high word is below 2^30): the four middle-column
+// products then sum below 2^64 with no carry, the low column a0 b0 + c0 d0
+// carries once into the high word, and the high column starts at p, which
+// leaves E = T_hi + p for the reduction of mont_dev.  Carries are read at
+// least two instructions after they are written, counting only
+// instructions inside these blocks.
+__device__ __forceinline__ u64 mont2_dev(const Prime& P, u64 a, u64 b, u64 c, u64 d) {
+    u64 L, M, H, cL, sx;
+    asm("v_mad_u64_u32 %[L], %[sx], %[a0], %[b0], 0\n\t"
+        "v_mad_u64_u32 %[M], %[sx], %[a1], %[b0], 0\n\t"
+        "v_mad_u64_u32 %[L], %[cL], %[c0], %[d0], %[L]\n\t"
+        "v_mad_u64_u32 %[H], %[sx], %[a1], %[b1], %[pp]\n\t"
+        "v_mad_u64_u32 %[M], %[sx], %[a0], %[b1], %[M]\n\t"
+        "v_mad_u64_u32 %[H], %[sx], %[c1], %[d1], %[H]\n\t"
+        "v_mad_u64_u32 %[M], %[sx], %[c1], %[d0], %[M]\n\t"
+        "v_mad_u64_u32 %[M], %[sx], %[c0], %[d1], %[M]"
+        : [L] "=&v"(L), [M] "=&v"(M), [H] "=&v"(H), [cL] "=&s"(cL), [sx] "=&s"(sx)
+        : [a0] "v"(lo32(a)), [a1] "v"(hi32(a)), [b0] "v"(lo32(b)), [b1] "v"(hi32(b)), [c0] "v"(lo32(c)),
+          [c1] "v"(hi32(c)), [d0] "v"(lo32(d)), [d1] "v"(hi32(d)), [pp] "s"(P.p));
+    // T = L + M 2^32 + (H - p) 2^64, L < 2^65 (carry cL), M < 2^64
+    u32 W1, H0, E0, E1p, E1, mh, shh, dq;
+    u64 G, c1, ch, ce, cq, s2;
+    asm("v_addc_co_u32_e64 %[H0], %[ch], %[h0], 0, %[cL]\n\t"
+        "v_add_co_u32_e64 %[W1], %[c1], %[l1], %[m0]\n\t"
+        "v_mad_u64_u32 %[G], %[s2], %[l0], %[k24], 0\n\t"
+        "v_addc_co_u32_e64 %[E1p], %[s2], %[h1], 0, %[ch]\n\t"
+        "v_addc_co_u32_e64 %[E0], %[ce], %[H0], %[m1], %[c1]"
+        : [W1] "=&v"(W1), [H0] "=&v"(H0), [E0] "=&v"(E0), [E1p] "=&v"(E1p), [G] "=&v"(G), [c1] "=&s"(c1),
+          [ch] "=&s"(ch), [ce] "=&s"(ce), [s2] "=&s"(s2)
+        : [l0] "v"(lo32(L)), [l1] "v"(hi32(L)), [m0] "v"(lo32(M)), [m1] "v"(hi32(M)), [h0] "v"(lo32(H)),
+          [h1] "v"(hi32(H)), [cL] "s"(cL), [k24] "s"(P.k24));
+    asm("v_sub_u32 %[mh], %[W1], %[g0]\n\t"
+        "v_and_b32 %[shh], 0xff000000, %[mh]\n\t"
+        "v_add_co_u32_e64 %[dq], %[cq], %[g0], %[shh]\n\t"
+        "v_addc_co_u32_e64 %[E1], %[s2], %[E1p], 0, %[ce]"
+        : [mh] "=&v"(mh), [shh] "=&v"(shh), [dq] "=&v"(dq), [E1] "=&v"(E1), [cq] "=&s"(cq), [s2] "=&s"(s2)
+        : [W1] "v"(W1), [g0] "v"(lo32(G)), [E1p] "v"(E1p), [ce] "s"(ce));
+    // W1 = T bits 32-63, E = T_hi + p; then as mont_dev
+    u64 F, dF;
+    asm("v_mad_u64_u32 %[F], %[dF], %[mh], %[nk], %[E]"
+        : [F] "=&v"(F), [dF] "=&s"(dF)
+        : [mh] "v"(mh), [nk] "s"(0u - P.k24), [E] "v"(mk64(E0, E1)));
     u32 T0, T1;
     u64 b1, b2;
     asm("v_subb_co_u32_e64 %[T0], %[b1], %[f0], %[q0], %[cq]\n\t"
@@ -140,16 +211,28 @@ __host__ __device__ __forceinline__ u64 mont2(const Prime& P, u64 a, u64 b, u64 
     return redc(P, lo, hi);
 }
 
-// x in [0, 2p) -> [0, p)
+// x in [0, 2p) -> [0, p): x - p (mod 2^64) is negative as a signed value
+// exactly when x < p (p < 2^63), and its sign is the sign of its high word.
+// On the device the subtraction is one 64-bit add of -p (v_lshl_add_u64
+// with the scalar pair) and the test a 32-bit compare.
+// (The empty asm hides the constant from the optimizer, which would
+// otherwise turn the add back into a two-instruction subtraction and a
+// 64-bit compare.)
+__host__ __device__ __forceinline__ u64 opaque_s(u64 v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+s"(v));
+#endif
+    return v;
+}
 __host__ __device__ __forceinline__ u64 csub(const Prime& P, u64 x) {
-    u64 y = x - P.p;
-    return (i64)y < 0 ? x : y;
+    u64 y = x + opaque_s(P.np);
+    return (int32_t)hi32(y) < 0 ? x : y;
 }
 
 // x in [0, 4p) -> [0, 2p) (requires 2p < 2^63)
 __host__ __device__ __forceinline__ u64 csub2(const Prime& P, u64 x) {
-    u64 y = x - P.p2;
-    return (i64)y < 0 ? x : y;
+    u64 y = x + opaque_s(P.n2p);
+    return (int32_t)hi32(y) < 0 ? x : y;
 }
 
 // Lazy butterflies after Harvey ("Faster arithmetic for number-theoretic
@@ -157,16 +240,28 @@ __host__ __device__ __forceinline__ u64 csub2(const Prime& P, u64 x) {
 // Montgomery products of inputs below 4p stay below 2p, so one conditional
 // subtraction of 2p per butterfly replaces two of p.
 // Forward: (x, y) in [0, 4p) -> (x + w*y, x - w*y) in [0, 4p).
+// On the device a = csub2(x) and p are the Montgomery product's addend, so
+// the reduction yields x' = a + t directly (t = w*y in (0, 2p)) and
+// y' = 2(a + p) - x' = a + 2p - t.  The addend bound holds: y1*w1 <
+// 4p^2/2^64 < 2^61.8 and a + p < 3p, so y1*w1 + a + p < 2^64; T_hi + a + p
+// < 4p < 2^64.
 __host__ __device__ __forceinline__ void ct_bfly_lazy(const Prime& P, u64& x, u64& y, u64 wm) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const u64 ap = csub2(P, x) + P.p;
+    const u64 X = mont_dev_add<true>(P, y, wm, ap);  // a + t
+    y = (ap << 1) - X;
+    x = X;
+#else
     u64 t = mont(P, y, wm);  // y*w < 4p*p < p*2^64: t in [0, 2p)
     u64 a = csub2(P, x);
     x = a + t;
     y = a + P.p2 - t;
+#endif
 }
 // Inverse: (x, y) in [0, 2p) -> (x + y, (x - y) * w) in [0, 2p).
 __host__ __device__ __forceinline__ void gs_bfly_lazy(const Prime& P, u64& x, u64& y, u64 wm) {
     u64 a = csub2(P, x + y);
-    u64 d = x + P.p2 - y;  // (0, 4p)
+    u64 d = (x + P.p2) - y;  // (0, 4p)
     y = mont(P, d, wm);
     x = a;
 }
@@ -177,12 +272,12 @@ __host__ __device__ __forceinline__ void ct_bfly(const Prime& P, u64& x, u64& y,
     u64 t = csub(P, mont(P, y, wm));
     u64 a = csub(P, x);
     x = a + t;
-    y = a + P.p - t;
+    y = (a + P.p) - t;
 }
 // Gentleman-Sande butterfly (x, y) -> (x + y, (x - y) * w), canonical in/out.
 __host__ __device__ __forceinline__ void gs_bfly(const Prime& P, u64& x, u64& y, u64 wm) {
     u64 a = csub(P, x + y);
-    u64 d = x + P.p - y;
+    u64 d = (x + P.p) - y;
     y = csub(P, mont(P, d, wm));
     x = a;
 }
@@ -190,8 +285,9 @@ __host__ __device__ __forceinline__ void gs_bfly(const Prime& P, u64& x, u64& y,
 // Base multiply of one 4-word group (reference asmNttMul0/1/2,
 // asmLinux.asm:910-1367): (x0+x1X)(y0+y1X) mod (X^2 - w) and
 // (x2+x3X)(y2+y3X) mod (X^2 + w).  Inputs in [0, 2p), or [0, 4p) when LAZY;
-// outputs canonical and carrying a factor 2^-64 (absorbed by the load
-// scaling).
+// outputs carry a factor 2^-64 (absorbed by the load scaling) and are
+// canonical, or in (0, 2p) when LAZY on the device (the lazy inverse levels
+// accept that).
 template <bool LAZY = false>
 __host__ __device__ __forceinline__ void base_mul4(const Prime& P, u64& x0, u64& x1, u64& x2, u64& x3, u64 y0,
                                                    u64 y1, u64 y2, u64 y3, u64 wm) {
@@ -201,12 +297,22 @@ __host__ __device__ __forceinline__ void base_mul4(const Prime& P, u64& x0, u64&
     }
     x0 = csub(P, x0); x1 = csub(P, x1); x2 = csub(P, x2); x3 = csub(P, x3);
     y0 = csub(P, y0); y1 = csub(P, y1); y2 = csub(P, y2); y3 = csub(P, y3);
-    u64 t1 = csub(P, mont(P, y1, wm));        // w*y1
-    u64 t3 = P.p - csub(P, mont(P, y3, wm));  // -w*y3, in (0, p]
-    u64 z0 = mont2(P, x0, y0, x1, t1);
-    u64 z1 = mont2(P, x0, y1, x1, y0);
-    u64 z2 = mont2(P, x2, y2, x3, t3);
-    u64 z3 = mont2(P, x2, y3, x3, y2);
+    const u64 t1 = csub(P, mont(P, y1, wm));        // w*y1
+    const u64 t3 = P.p - csub(P, mont(P, y3, wm));  // -w*y3, in (0, p]
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (LAZY) {
+        const u64 z0 = mont2_dev(P, x0, y0, x1, t1);
+        const u64 z1 = mont2_dev(P, x0, y1, x1, y0);
+        const u64 z2 = mont2_dev(P, x2, y2, x3, t3);
+        const u64 z3 = mont2_dev(P, x2, y3, x3, y2);
+        x0 = z0; x1 = z1; x2 = z2; x3 = z3;
+        return;
+    }
+#endif
+    const u64 z0 = mont2(P, x0, y0, x1, t1);
+    const u64 z1 = mont2(P, x0, y1, x1, y0);
+    const u64 z2 = mont2(P, x2, y2, x3, t3);
+    const u64 z3 = mont2(P, x2, y3, x3, y2);
     x0 = csub(P, z0); x1 = csub(P, z1); x2 = csub(P, z2); x3 = csub(P, z3);
 }
 
