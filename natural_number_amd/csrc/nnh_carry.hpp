@@ -33,15 +33,27 @@ struct CrtArgs {
 
 constexpr int CRT_TILE_ROUNDS = 16;  // 16 x 256 positions per tile
 
-__device__ __forceinline__ void crt_x(const CrtArgs& A, u64 k, u64& x0, u64& x1, u64& x2) {
+// The three words at position k as loaded (residues, or column words);
+// zero past n_res.
+struct CrtIn {
+    u64 r0, r1, r2;
+};
+__device__ __forceinline__ CrtIn crt_load(const CrtArgs& A, u64 k) {
+    if (k >= A.n_res) return CrtIn{0, 0, 0};
+    return CrtIn{A.r[0][k], A.r[1][k], A.r[2][k]};
+}
+__device__ __forceinline__ void crt_finish(const CrtArgs& A, u64 k, const CrtIn& in, u64& x0, u64& x1, u64& x2) {
     if (k >= A.n_res) x0 = x1 = x2 = 0;
-    else if (A.columns) { x0 = A.r[0][k]; x1 = A.r[1][k]; x2 = A.r[2][k]; }
+    else if (A.columns) { x0 = in.r0; x1 = in.r1; x2 = in.r2; }
     else {
         // p0 and p1 leave the inverse transform in [0, 2p) (lazy butterflies)
         constexpr Prime Q0 = prime_of(0), Q1 = prime_of(1);
-        garner(csub(Q0, A.r[0][k]), csub(Q1, A.r[1][k]), A.r[2][k], A.C.c02m, A.C.m0m, A.C.c1m, A.C.m02lo,
-               A.C.m02hi, x0, x1, x2);
+        garner(csub(Q0, in.r0), csub(Q1, in.r1), in.r2, A.C.c02m, A.C.m0m, A.C.c1m, A.C.m02lo, A.C.m02hi, x0, x1,
+               x2);
     }
+}
+__device__ __forceinline__ void crt_x(const CrtArgs& A, u64 k, u64& x0, u64& x1, u64& x2) {
+    crt_finish(A, k, crt_load(A, k), x0, x1, x2);
 }
 
 __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn_slot, u64 (*xs)[256 + 3],
@@ -55,11 +67,16 @@ __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn
         xs[0][t] = x0; xs[1][t] = x1; xs[2][t] = x2;
     }
     u32 cin0 = 0, cin1 = 1;  // running carry assuming tile carry-in 0 / 1
+    // each round's words are loaded one round ahead, so that their HBM
+    // latency overlaps the previous round's reconstruction and carries
+    CrtIn next = crt_load(A, tile_base + t);
     for (int r = 0; r < CRT_TILE_ROUNDS; r++) {
         const u64 k = tile_base + (u64)r * 256 + t;
         if (k >= A.n_out && tile_base + (u64)r * 256 >= A.n_out) break;  // uniform
+        const CrtIn cur = next;
+        if (r + 1 < CRT_TILE_ROUNDS) next = crt_load(A, k + 256);
         u64 x0, x1, x2;
-        crt_x(A, k, x0, x1, x2);
+        crt_finish(A, k, cur, x0, x1, x2);
         __syncthreads();
         xs[0][3 + t] = x0; xs[1][3 + t] = x1; xs[2][3 + t] = x2;
         __syncthreads();
@@ -115,10 +132,13 @@ __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn
 // Single work-group scan of tile carry functions -> carry-in per tile.
 // Each tile function is kill (00), generate (11) or propagate (10).  cin0 is
 // the carry into the first tile; the carry out of the last is stored to
-// *cout when cout is non-null.
+// *cout when cout is non-null.  1024 threads: each composes a run of
+// consecutive tiles, a wave resolves its 64 runs with one 64-bit add of
+// its generate/propagate ballots (as the CRT does for words), and the 16
+// wave functions are composed in order.
 __device__ __forceinline__ void carry_scan_body(const u8* tile_fn, u64 n_tiles, u8* tile_cin, u32* chunk_fn,
                                                 u32 cin0 = 0, u32* cout = nullptr) {
-    const u32 t = threadIdx.x;
+    const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const u64 per = (n_tiles + 1023) / 1024;
     const u64 b0 = (u64)t * per, b1 = b0 + per < n_tiles ? b0 + per : n_tiles;
     u32 f0 = 0, f1 = 1;  // composed function of this thread's tiles
@@ -128,20 +148,25 @@ __device__ __forceinline__ void carry_scan_body(const u8* tile_fn, u64 n_tiles, 
         f0 = f0 ? o1 : o0;
         f1 = f1 ? o1 : o0;
     }
-    chunk_fn[t] = f0 | (f1 << 1);
+    const u64 G = __ballot(f0 != 0), P = __ballot(f0 == 0 && f1 != 0), Am = G | P;
+    const u64 s0 = Am + G, s1 = s0 + 1;
+    const u32 co0 = s0 < Am ? 1u : 0u, co1 = co0 | (s1 == 0 ? 1u : 0u);
+    if (lane == 0) chunk_fn[wv] = co0 | (co1 << 1);
     __syncthreads();
-    // serial exclusive scan over 1024 thread functions (cheap: 1024 steps)
-    if (t == 0) {
+    u32 cw = cin0;  // carry into this wave
+    for (u32 w = 0; w < wv; w++) {
+        const u32 f = chunk_fn[w];
+        cw = cw ? (f >> 1) & 1u : f & 1u;
+    }
+    if (cout && t == 0) {
         u32 c = cin0;
-        for (u32 i = 0; i < 1024; i++) {
-            u32 f = chunk_fn[i];
-            chunk_fn[i] = c;
+        for (u32 w = 0; w < 16; w++) {
+            const u32 f = chunk_fn[w];
             c = c ? (f >> 1) & 1u : f & 1u;
         }
-        if (cout) *cout = c;
+        *cout = c;
     }
-    __syncthreads();
-    u32 c = chunk_fn[t];
+    u32 c = (u32)((((cw ? s1 : s0) ^ P) >> lane) & 1u);
     for (u64 i = b0; i < b1; i++) {
         tile_cin[i] = (u8)c;
         u32 f = tile_fn[i];
