@@ -69,6 +69,13 @@ struct Buf {
 struct Device {
     int id = -1;
     hipStream_t stream = nullptr;
+    // The transform stages of p2 run on stream2 beside those of p0/p1 on the
+    // caller's stream (the prime chains are independent until the CRT), so
+    // one chain's memory-heavy top passes overlap the other's VALU-bound
+    // chunk kernel and the launch tails overlap.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool two_streams = true;
     RootBases rb{};
     CrtConsts crt{};
     u64 rb_plain[3][56]{}, irb_plain[3][56]{};
@@ -216,6 +223,13 @@ static int device_init(int dev) {
         return fail(NNH_ERR_NODEV, "no HIP device " + std::to_string(dev));
     HIPCHK(hipSetDevice(dev));
     if (!g_dev.stream) HIPCHK(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+    if (!g_dev.stream2) HIPCHK(hipStreamCreateWithFlags(&g_dev.stream2, hipStreamNonBlocking));
+    if (!g_dev.ev_fork) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_fork, hipEventDisableTiming));
+    if (!g_dev.ev_join) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_join, hipEventDisableTiming));
+    {
+        const char* e = std::getenv("NNH_ONE_STREAM");
+        g_dev.two_streams = !(e && e[0] == '1');
+    }
     g_dev.id = dev;
     build_constants(g_dev);
     g_inited = true;
@@ -263,8 +277,28 @@ static void set3(u64* dst[3], u64* base, u64 L) {
     for (int i = 0; i < 3; i++) dst[i] = base + (u64)i * L;
 }
 
+// Stream for the p2 stages: stream2 after it has waited for everything
+// queued on st so far, or st itself (one stream while per-kernel timing is
+// on, so that the events bracket single kernels).
+static int fork_p2(Device& D, hipStream_t st, hipStream_t* st2) {
+    *st2 = st;
+    if (!D.two_streams || g_prof || st == D.stream2) return NNH_OK;
+    HIPCHK(hipEventRecord(D.ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(D.stream2, D.ev_fork, 0));
+    *st2 = D.stream2;
+    return NNH_OK;
+}
+// st waits for the p2 stages queued on st2.
+static int join_p2(Device& D, hipStream_t st, hipStream_t st2) {
+    if (st2 == st) return NNH_OK;
+    HIPCHK(hipEventRecord(D.ev_join, st2));
+    HIPCHK(hipStreamWaitEvent(st, D.ev_join, 0));
+    return NNH_OK;
+}
+
 // Forward top levels (global bits s-1 .. 12) of operand x into f.
-static void fwd_top(const Device& D, const Plan& pl, const u64* x, u64 xn, u64* f, hipStream_t st) {
+static void fwd_top(const Device& D, const Plan& pl, const u64* x, u64 xn, u64* f, hipStream_t st,
+                    hipStream_t st2) {
     bool first = true;
     for (auto [g_lo, nl] : top_passes(pl)) {
         TopPassDesc td{};
@@ -283,14 +317,14 @@ static void fwd_top(const Device& D, const Plan& pl, const u64* x, u64 xn, u64* 
             // algorithmic bytes: limbs read once (first pass) or residues read, residues written
             u64 rd = first ? 8 * std::min<u64>(xn, pl.L) : 24 * pl.L;
             ProfScope ps(first ? K_TOP_LOAD : K_TOP_FWD, rd + 24 * pl.L, st);
-            launch_top_pass(td, st);
+            launch_top_pass(td, st, st2);
         }
         first = false;
     }
 }
 
 // Inverse top levels (global bits 12 .. s-1), mirrored passes.
-static void inv_top(const Device& D, const Plan& pl, u64* f, hipStream_t st) {
+static void inv_top(const Device& D, const Plan& pl, u64* f, hipStream_t st, hipStream_t st2) {
     std::vector<std::pair<int, int>> passes = top_passes(pl);
     for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
         TopPassDesc td{};
@@ -306,7 +340,7 @@ static void inv_top(const Device& D, const Plan& pl, u64* f, hipStream_t st) {
         td.inverse = true;
         {
             ProfScope ps(K_TOP_INV, 48 * pl.L, st);
-            launch_top_pass(td, st);
+            launch_top_pass(td, st, st2);
         }
     }
 }
@@ -314,7 +348,8 @@ static void inv_top(const Device& D, const Plan& pl, u64* f, hipStream_t st) {
 // Fused chunk kernel.  a/b are limbs when the transform fits one chunk and
 // the operand is not yet transformed.
 static void mid(const Device& D, const Plan& pl, u64* fa, const u64* fb, const u64* la, u64 an,
-                const u64* lb, u64 bn, bool sqr, bool fwd_only, bool a_done, bool b_done, hipStream_t st) {
+                const u64* lb, u64 bn, bool sqr, bool fwd_only, bool a_done, bool b_done, hipStream_t st,
+                hipStream_t st2) {
     MidDesc md{};
     set3(md.fa, fa, pl.L);
     for (int i = 0; i < 3; i++) {
@@ -337,7 +372,7 @@ static void mid(const Device& D, const Plan& pl, u64* fa, const u64* fb, const u
     u64 ra = la ? 8 * std::min<u64>(an, pl.L) : 24 * pl.L;
     u64 rb = fwd_only || sqr ? 0 : (lb ? 8 * std::min<u64>(bn, pl.L) : 24 * pl.L);
     ProfScope ps(K_MID, ra + rb + 24 * pl.L, st);
-    launch_mid(md, st);
+    launch_mid(md, st, st2);
 }
 
 static int crt(Device& D, const Plan& pl, const u64* f, u64 n_res, u64* out, u64 n_out, const u64* addend,
@@ -372,22 +407,30 @@ static int product_core(Device& D, const Plan& pl, const u64* a, u64 an, const u
     u64* fb = D.ws.p + 3 * pl.L;
     const u64* fbp = fb_done ? fb_done : fb;
     bool small = pl.ntop == 0;
+    hipStream_t st2;
+    int rc = fork_p2(D, st, &st2);
+    if (rc) return rc;
     if (!small) {
-        fwd_top(D, pl, a, an, fa, st);
-        if (!sqr && !fb_done) fwd_top(D, pl, b, bn, fb, st);
+        fwd_top(D, pl, a, an, fa, st, st2);
+        if (!sqr && !fb_done) fwd_top(D, pl, b, bn, fb, st, st2);
     }
     mid(D, pl, fa, fbp, small ? a : nullptr, an, (small && !sqr && !fb_done) ? b : nullptr, bn, sqr, false,
-        false, fb_done != nullptr, st);
-    if (!small) inv_top(D, pl, fa, st);
+        false, fb_done != nullptr, st, st2);
+    if (!small) inv_top(D, pl, fa, st, st2);
+    if ((rc = join_p2(D, st, st2))) return rc;
     u64 n_res = std::min<u64>(an + (sqr ? an : bn) - 1, pl.L);
     return crt(D, pl, fa, n_res, out, n_out, addend, n_add, st);
 }
 
 // Complete forward transform of x into dst (3 L words).
-static void transform_full(Device& D, const Plan& pl, const u64* x, u64 xn, u64* dst, hipStream_t st) {
+static int transform_full(Device& D, const Plan& pl, const u64* x, u64 xn, u64* dst, hipStream_t st) {
     bool small = pl.ntop == 0;
-    if (!small) fwd_top(D, pl, x, xn, dst, st);
-    mid(D, pl, dst, dst, small ? x : nullptr, xn, nullptr, 0, false, true, false, false, st);
+    hipStream_t st2;
+    int rc = fork_p2(D, st, &st2);
+    if (rc) return rc;
+    if (!small) fwd_top(D, pl, x, xn, dst, st, st2);
+    mid(D, pl, dst, dst, small ? x : nullptr, xn, nullptr, 0, false, true, false, false, st, st2);
+    return join_p2(D, st, st2);
 }
 
 static int school(Device& D, const u64* a, u64 an, const u64* b, u64 bn, u64* out, hipStream_t st) {
@@ -444,7 +487,7 @@ static int mul_device(Device& D, u64* out, const u64* a, u64 an, const u64* b, u
     if ((rc = ensure_twiddles(D, s_opt)) || (rc = ensure(D.ws, 6 * pl.L)) || (rc = ensure(D.tmp, bn + 64)))
         return rc;
     u64* fb = D.ws.p + 3 * pl.L;
-    transform_full(D, pl, b, bn, fb, st);
+    if ((rc = transform_full(D, pl, b, bn, fb, st))) return rc;
     // low half: a0 * b -> out[0 .. a0n + bn)
     if ((rc = product_core(D, pl, a, a0n, b, bn, false, fb, out, a0n + bn, nullptr, 0, st))) return rc;
     // high half: a1 * b + (overlap) -> out[a0n .. an + bn)
@@ -514,7 +557,7 @@ static int ntt_chunked(Device& D, u64* out, const u64* a, u64 an, const u64* b, 
     int rc;
     if ((rc = ensure_twiddles(D, sp)) || (rc = ensure(D.ws, 6 * pl.L)) || (rc = ensure(D.tmp, bn + 64))) return rc;
     u64* fb = D.ws.p + 3 * pl.L;
-    transform_full(D, pl, b, bn, fb, st);
+    if ((rc = transform_full(D, pl, b, bn, fb, st))) return rc;
     for (u64 off = 0; off < an; off += P) {
         u64 pn = std::min(P, an - off);
         if (off == 0) {
@@ -542,7 +585,7 @@ static int ntt_refsplit(Device& D, u64* out, const u64* a, u64 an, const u64* b,
     int rc;
     if ((rc = ensure_twiddles(D, s_opt)) || (rc = ensure(D.ws, 6 * pl.L)) || (rc = ensure(D.tmp, bn + 64))) return rc;
     u64* fb = D.ws.p + 3 * pl.L;
-    transform_full(D, pl, b, bn, fb, st);
+    if ((rc = transform_full(D, pl, b, bn, fb, st))) return rc;
     if ((rc = product_core(D, pl, a, a0n, b, bn, false, fb, out, a0n + bn, nullptr, 0, st))) return rc;
     HIPCHK(hipMemcpyAsync(D.tmp.p, out + a0n, bn * sizeof(u64), hipMemcpyDeviceToDevice, st));
     return product_core(D, pl, a + a0n, an - a0n, b, bn, false, fb, out + a0n, an - a0n + bn, D.tmp.p, bn, st);
@@ -1189,7 +1232,7 @@ int nnh_transform_forward(uint64_t* out, const uint64_t* a, uint64_t an, int sca
         return rc;
     hipStream_t st = D.stream;
     HIPCHK(hipMemcpyAsync(D.stage.p, a, an * 8, hipMemcpyHostToDevice, st));
-    transform_full(D, pl, D.stage.p, an, D.ws.p, st);
+    if ((rc = transform_full(D, pl, D.stage.p, an, D.ws.p, st))) return rc;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, D.ws.p, 3 * pl.L * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));

@@ -109,8 +109,9 @@ void launch_crt_batch(const BJob* jobs, u32 njobs, u64 ntiles, u8* tile_fn, u8* 
                       hipStream_t st);
 
 void launch_twiddles(u64* tw, u64 n, const RootBases& rb, int pr, int inv, hipStream_t st);
-void launch_top_pass(const TopPassDesc& d, hipStream_t st);
-void launch_mid(const MidDesc& d, hipStream_t st);
+// The p0/p1 kernels go to st, the p2 kernels to st2 (st when null).
+void launch_top_pass(const TopPassDesc& d, hipStream_t st, hipStream_t st2 = nullptr);
+void launch_mid(const MidDesc& d, hipStream_t st, hipStream_t st2 = nullptr);
 void launch_crt(const CrtDesc& d, hipStream_t st);
 u64 crt_tiles(u64 n_out);
 void launch_school_cols(const u64* a, u64 an, const u64* b, u64 bn, u64* c0, u64* c1, u64* c2, hipStream_t st);

@@ -8,9 +8,9 @@ void launch_mid_part(const MidDesc& d, bool lazy, hipStream_t st);
 void launch_mid_batch_part(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs,
                            bool fwd_only, bool lazy, hipStream_t st);
 
-void launch_mid(const MidDesc& D, hipStream_t st) {
+void launch_mid(const MidDesc& D, hipStream_t st, hipStream_t st2) {
     launch_mid_part(D, true, st);
-    mid_primes<false>(D, st);
+    mid_primes<false>(D, st2 ? st2 : st);
 }
 
 void launch_mid_batch(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs, bool fwd_only,

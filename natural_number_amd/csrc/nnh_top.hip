@@ -8,9 +8,9 @@ void launch_top_pass_part(const TopPassDesc& d, bool lazy, hipStream_t st);
 void launch_top_pass_batch_part(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
                                 bool from_limbs, bool lazy, hipStream_t st);
 
-void launch_top_pass(const TopPassDesc& D, hipStream_t st) {
+void launch_top_pass(const TopPassDesc& D, hipStream_t st, hipStream_t st2) {
     launch_top_pass_part(D, true, st);
-    top_pass_primes<false>(D, st);
+    top_pass_primes<false>(D, st2 ? st2 : st);
 }
 
 void launch_top_pass_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
