@@ -57,6 +57,24 @@ __device__ __forceinline__ void xchg(u64 (&v)[16], u64* lds, u32 t) {
     __syncthreads();
     CFor<0, 16>::run([&](auto jc) { v[decltype(jc)::value] = lds[bt + padj<decltype(jc)::value, LT>()]; });
 }
+// The same transpose through half the LDS: low words, then high words.
+// Kernels whose occupancy the 64-bit buffer limits (the top passes) use it.
+template <int CB, int LF, int LT>
+__device__ __forceinline__ void xchg(u64 (&v)[16], u32* lds, u32 t) {
+    const u32 bf = pad(lay_t(t, LF)), bt = pad(lay_t(t, LT));
+    u32 lo[16];
+    __syncthreads();
+    CFor<0, 16>::run([&](auto jc) { lds[bf + padj<decltype(jc)::value, LF>()] = lo32(v[decltype(jc)::value]); });
+    __syncthreads();
+    CFor<0, 16>::run([&](auto jc) { lo[decltype(jc)::value] = lds[bt + padj<decltype(jc)::value, LT>()]; });
+    __syncthreads();
+    CFor<0, 16>::run([&](auto jc) { lds[bf + padj<decltype(jc)::value, LF>()] = hi32(v[decltype(jc)::value]); });
+    __syncthreads();
+    CFor<0, 16>::run([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        v[j] = mk64(lo[j], lds[bt + padj<j, LT>()]);
+    });
+}
 
 // One level at local bit L with the register window at LO (L-LO in 0..3).
 // Block index of element y: b = tile_id 2^(CB-L-1) + (y >> (L+1)); for the
@@ -94,8 +112,8 @@ __host__ __device__ constexpr int inv_window(int l, int l_lo, int lo0, int cb) {
     return lo;
 }
 
-template <int CB, int L, int L_LO, int L_HI, int LO0, int LO_MIN, bool LAZY>
-__device__ __forceinline__ void fwd_rec(const Prime& P, u64 (&v)[16], u64* lds, u32 t, u64 tile_id, const u64* __restrict__ tw) {
+template <int CB, int L, int L_LO, int L_HI, int LO0, int LO_MIN, bool LAZY, class LT>
+__device__ __forceinline__ void fwd_rec(const Prime& P, u64 (&v)[16], LT* lds, u32 t, u64 tile_id, const u64* __restrict__ tw) {
     if constexpr (L >= L_LO) {
         constexpr int prev = L == L_HI ? LO0 : fwd_window(L + 1, L_HI, LO0, LO_MIN);
         constexpr int cur = fwd_window(L, L_HI, LO0, LO_MIN);
@@ -104,8 +122,8 @@ __device__ __forceinline__ void fwd_rec(const Prime& P, u64 (&v)[16], u64* lds, 
         fwd_rec<CB, L - 1, L_LO, L_HI, LO0, LO_MIN, LAZY>(P, v, lds, t, tile_id, tw);
     }
 }
-template <int CB, int L, int L_LO, int L_HI, int LO0, bool LAZY>
-__device__ __forceinline__ void inv_rec(const Prime& P, u64 (&v)[16], u64* lds, u32 t, u64 tile_id, const u64* __restrict__ tw) {
+template <int CB, int L, int L_LO, int L_HI, int LO0, bool LAZY, class LT>
+__device__ __forceinline__ void inv_rec(const Prime& P, u64 (&v)[16], LT* lds, u32 t, u64 tile_id, const u64* __restrict__ tw) {
     if constexpr (L <= L_HI) {
         constexpr int prev = L == L_LO ? LO0 : inv_window(L - 1, L_LO, LO0, CB);
         constexpr int cur = inv_window(L, L_LO, LO0, CB);
@@ -159,8 +177,8 @@ struct TopArgs {
 
 // Top pass over a 4096-element tile of Q = 2^(12-NLEV) consecutive columns
 // by R = 2^NLEV rows of stride d = 2^g_lo (local index y = col + Q row).
-template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY>
-__device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile, u64* lds) {
+template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY, class LT>
+__device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile, LT* lds) {
     const Prime P = make_prime(PRIME_K[0] * (pr == 0) + PRIME_K[1] * (pr == 1) + PRIME_K[2] * (pr == 2));
     constexpr int CB = 12, Q = 12 - NLEV;
     constexpr u32 QM = (1u << Q) - 1u;
@@ -168,6 +186,9 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
     constexpr int LO_INV0 = Q < CB - 4 ? Q : CB - 4;        // inverse starts at the bottom row bit
     constexpr int LO_END = INV ? inv_window(CB - 1, Q, LO_INV0, CB) : fwd_window(Q, CB - 1, LO_IO, Q);
     constexpr int LO_ST = LO_END < Q ? LO_IO : LO_END;      // store window (columns in lane bits)
+    // The inverse loads straight into its first window: lanes still cover
+    // 2^min(Q, 8) consecutive columns of a row, as in LO_IO.
+    constexpr int LO_LD = INV ? LO_INV0 : LO_IO;
     const u32 t = threadIdx.x;
     const u64 d = 1ull << A.g_lo;
     const u64 tiles_per_blk = d >> Q;
@@ -184,9 +205,9 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
     // (Uniform per work-group.)
     const bool top_copy = FROM_LIMBS && !INV && blk == 0 && (d << (NLEV - 1)) >= A.an;
     {
-        const u32 yt = lay_t(t, LO_IO);
+        const u32 yt = lay_t(t, LO_LD);
         auto xs = [&](int j) {
-            const u32 y = yt | ((u32)j << LO_IO);
+            const u32 y = yt | ((u32)j << LO_LD);
             return xbase + (y & QM) + (u64)(y >> Q) * d;
         };
         if constexpr (FROM_LIMBS) {
@@ -206,7 +227,6 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
             fwd_rec<CB, CB - 1, Q, CB - 1, LO_IO, Q, LAZY>(P, v, lds, t, blk, tw);
         }
     } else {
-        if constexpr (LO_INV0 != LO_IO) xchg<CB, LO_IO, LO_INV0>(v, lds, t);
         inv_rec<CB, Q, Q, CB - 1, LO_INV0, LAZY>(P, v, lds, t, blk, tw);
     }
     if constexpr (LO_ST != LO_END) xchg<CB, LO_END, LO_ST>(v, lds, t);

@@ -10,7 +10,7 @@ namespace nnh {
 
 template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY>
 __global__ __launch_bounds__(256) void k_top_pass(TopArgs A, u32 pr0) {
-    __shared__ u64 lds[4096 + 256];
+    __shared__ u32 lds[4096 + 256];  // 32-bit transposes: half the LDS, more work-groups per CU
     top_pass_body<INV, FROM_LIMBS, NLEV, LAZY>(A, pr0 + blockIdx.y, blockIdx.x, lds);
 }
 
@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void k_top_pass(TopArgs A, u32 pr0) {
 template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY>
 __global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__ jobs, u32 njobs, int operand,
                                                         TopBatchParams P, u32 pr0) {
-    __shared__ u64 lds[4096 + 256];
+    __shared__ u32 lds[4096 + 256];  // 32-bit transposes: half the LDS, more work-groups per CU
     const u64 tpj = P.L >> 12;
     const u32 pr = pr0 + blockIdx.y;
     const u64 tile = blockIdx.x % tpj;
