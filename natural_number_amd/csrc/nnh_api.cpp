@@ -33,7 +33,8 @@ constexpr u64 P[3] = {27ull << 56 | 1, 58ull << 56 | 1, 87ull << 56 | 1};
 // that table is a repeated square of it.
 constexpr u64 ROOT_GEN[3] = {11, 2, 310};
 constexpr int MIN_MID_BITS = 8;       // smallest fused-chunk transform (2^8 words)
-constexpr int TILE_BITS = 12;         // fused chunk / top-pass tile: 4096 words
+constexpr int TILE_BITS = 12;         // top-pass tile and largest fused chunk: 4096 words
+constexpr int MIN_CHUNK_BITS = 10;    // smallest fused chunk beside top passes (64 threads)
 constexpr int MAX_SCALE = 30;         // largest transform handled (2^30 words per prime)
 constexpr u64 NTT_MUL_THRESHOLD = 384;  // reference NTTmul_threshold (ntt_info.h:6)
 
@@ -242,16 +243,29 @@ static int device_init(int dev) {
 struct Plan {
     int s;
     u64 L;
-    int cb;          // fused chunk bits (min(s, 12))
+    int cb;          // fused chunk bits (mid_bits(s))
     int ntop;        // top levels (s - cb)
     u64 alpha_m[3];
 };
+
+// Fused chunk bits for a transform of 2^s words.  Above 2^20 words the
+// chunk kernel is VALU-bound and the top passes (at most 8 levels each, one
+// HBM round trip per pass) are HBM-bound with VALU to spare, so the top
+// passes take as many levels as the fewest passes allow, the chunk keeping
+// at least 2^10 words (one wave per work-group): 3.5 -> 3.4 ms for the
+// 2^30-bit product.  Smaller transforms (batch buckets) measured faster
+// with the largest chunk.
+static int mid_bits(int s) {
+    if (s <= 20) return std::min(s, TILE_BITS);
+    const int npass = (s - TILE_BITS + 7) / 8;
+    return std::max(MIN_CHUNK_BITS, s - 8 * npass);
+}
 
 static Plan make_plan(const Device& D, int s) {
     Plan pl;
     pl.s = s;
     pl.L = 1ull << s;
-    pl.cb = std::min(s, TILE_BITS);
+    pl.cb = mid_bits(s);
     pl.ntop = s - pl.cb;
     for (int i = 0; i < 3; i++) pl.alpha_m[i] = to_mont(alpha_plain(D, i, s), P[i]);
     return pl;
