@@ -30,7 +30,7 @@ HBM_PEAK = 8.0e12
 # HBM bytes per launch of the fused chunk kernel at this workload: rocprofv3
 # FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the counter's reading
 # for a plain copy of known size (profiles/ntt_product_2e30x2e30_pmc.md).
-MID_FUSED_TRAFFIC = 3.381e9  # HBM bytes per product of both chunk-kernel launches, rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated)
+MID_FUSED_TRAFFIC = 3.501e9  # HBM bytes per product of both chunk-kernel launches, rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated)
 
 
 def log(*a):
@@ -264,15 +264,19 @@ def valu_ceiling(nn, kernels, steps):
     ideal time.  Butterflies per product: three transforms per prime of 24
     levels of 2^24 butterflies (the 25th level is the base multiply modulo
     X^2 -+ w), less the load pass's top level, which is a copy (2 x 2^24 per
-    prime): load 2 x 6, forward 2 x 6, chunk kernel 2 x 11 + 11, inverse 13."""
+    prime).  The split between kernels is the library's plan for 2^25 words
+    (nnh_transform_plan): the first top pass loads the limbs, the chunk
+    kernel runs chunk_bits - 1 levels of each of the three transforms, the
+    inverse runs every top-pass level once."""
     log("butterfly issue ceiling")
     r_lazy = nn.butterfly_peak(1)
     r_strict = nn.butterfly_peak(2)
     half = 1 << 24
     # ideal seconds per butterfly-level over the three primes (two lazy, one strict)
     per_level = 2 * half / r_lazy + half / r_strict
-    levels = {"top_pass_load": 2 * 6, "top_pass_fwd": 2 * 6, "mid_fused": 2 * 11 + 11,
-              "top_pass_inv": 13, "crt_carry": 0}
+    cb, passes = nn.transform_plan(25)
+    levels = {"top_pass_load": 2 * (passes[0] - 1), "top_pass_fwd": 2 * sum(passes[1:]),
+              "mid_fused": 3 * (cb - 1), "top_pass_inv": sum(passes), "crt_carry": 0}
     out = {"peak_lazy_bfly_per_s": r_lazy, "peak_strict_bfly_per_s": r_strict, "kernels": {}}
     tot_ideal = tot_meas = 0.0
     for name, nl in levels.items():

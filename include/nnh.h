@@ -175,6 +175,15 @@ int nnh_kernel_times(int kind, double* total_ms, uint64_t* launches, double* byt
  * bounds them from above. */
 int nnh_butterfly_peak(int prime, uint64_t iters, double* butterflies_per_s);
 
+/* Level split of a transform of 2^scale words per prime (for benchmarks;
+ * the counterpart of the reference's cache blocking of NTT/INTT into global
+ * levels and 2^20-word chunks, ntt_info.h:187-196, 209-219): the fused
+ * chunk kernel's chunk is 2^chunk_bits words (it runs chunk_bits - 1 levels
+ * of each transform), and n_passes <= 4 strided top passes run
+ * pass_levels[i] levels each, in forward order (the inverse runs them in
+ * reverse).  Needs no device. */
+int nnh_transform_plan(int scale, int* chunk_bits, int* n_passes, int* pass_levels);
+
 /* Device plumbing (no counterpart in the reference): device memory, copies,
  * streams and events through the library's own HIP runtime, so that a
  * caller needs no second GPU runtime in its process.  Streams are created

@@ -67,6 +67,7 @@ def lib():
         "nnh_mul_algo_device_async": ([vp, vp, u64, vp, u64, i32, i32, vp], i32),
         "nnh_time_algo": ([vp, vp, u64, vp, u64, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_float)], i32),
         "nnh_butterfly_peak": ([i32, u64, ctypes.POINTER(ctypes.c_double)], i32),
+        "nnh_transform_plan": ([i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
         "nnh_mlp_create": ([i32, i32, u64, ctypes.POINTER(vp)], i32),
         "nnh_mlp_destroy": ([vp], i32),
         "nnh_mlp_num_params": ([vp], i32),
@@ -194,6 +195,14 @@ def sqr_device_async(d_out, d_a, an, stream=None):
 
 def kernel_timing(enable):
     _check(lib().nnh_kernel_timing(1 if enable else 0))
+
+
+def transform_plan(scale):
+    """(chunk_bits, [levels of each top pass, forward order]) of a 2^scale-word transform."""
+    cb, n = ctypes.c_int(), ctypes.c_int()
+    lv = (ctypes.c_int * 4)()
+    _check(lib().nnh_transform_plan(scale, ctypes.byref(cb), ctypes.byref(n), lv))
+    return cb.value, [lv[i] for i in range(n.value)]
 
 
 def butterfly_peak(prime, iters=256):

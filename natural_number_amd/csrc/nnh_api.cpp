@@ -1043,6 +1043,22 @@ int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an,
     return NNH_OK;
 }
 
+int nnh_transform_plan(int scale, int* chunk_bits, int* n_passes, int* pass_levels) {
+    if (scale < 2 || scale > MAX_SCALE || !chunk_bits || !n_passes || !pass_levels)
+        return fail(NNH_ERR_ARG, "bad transform-plan arguments");
+    Plan pl;
+    pl.s = scale;
+    pl.L = 1ull << scale;
+    pl.cb = mid_bits(scale);
+    pl.ntop = scale - pl.cb;
+    std::vector<std::pair<int, int>> passes = top_passes(pl);
+    if (passes.size() > 4) return fail(NNH_ERR_ARG, "more than four top passes");
+    *chunk_bits = pl.cb;
+    *n_passes = (int)passes.size();
+    for (size_t i = 0; i < passes.size(); i++) pass_levels[i] = passes[i].second;
+    return NNH_OK;
+}
+
 int nnh_butterfly_peak(int prime, uint64_t iters, double* butterflies_per_s) {
     if (prime < 0 || prime > 2 || iters < 1 || iters > (1u << 20) || !butterflies_per_s)
         return fail(NNH_ERR_ARG, "bad butterfly-peak arguments");

@@ -282,6 +282,29 @@ __host__ __device__ __forceinline__ void gs_bfly(const Prime& P, u64& x, u64& y,
     x = a;
 }
 
+// The same butterflies for twiddle 1 (block 0 of a level): no product;
+// w*y is y brought into the range the Montgomery product would give.
+__host__ __device__ __forceinline__ void ct_bfly_lazy_one(const Prime& P, u64& x, u64& y) {
+    const u64 a = csub2(P, x), t = csub2(P, y);
+    x = a + t;
+    y = (a + P.p2) - t;
+}
+__host__ __device__ __forceinline__ void gs_bfly_lazy_one(const Prime& P, u64& x, u64& y) {
+    const u64 a = csub2(P, x + y);
+    y = csub2(P, (x + P.p2) - y);
+    x = a;
+}
+__host__ __device__ __forceinline__ void ct_bfly_one(const Prime& P, u64& x, u64& y) {
+    const u64 a = csub(P, x), t = csub(P, y);
+    x = a + t;
+    y = (a + P.p) - t;
+}
+__host__ __device__ __forceinline__ void gs_bfly_one(const Prime& P, u64& x, u64& y) {
+    const u64 a = csub(P, x + y);
+    y = csub(P, (x + P.p) - y);
+    x = a;
+}
+
 // Base multiply of one 4-word group (reference asmNttMul0/1/2,
 // asmLinux.asm:910-1367): (x0+x1X)(y0+y1X) mod (X^2 - w) and
 // (x2+x3X)(y2+y3X) mod (X^2 + w).  Inputs in [0, 2p), or [0, 4p) when LAZY;

@@ -79,7 +79,11 @@ __device__ __forceinline__ void xchg(u64 (&v)[16], u32* lds, u32 t) {
 // One level at local bit L with the register window at LO (L-LO in 0..3).
 // Block index of element y: b = tile_id 2^(CB-L-1) + (y >> (L+1)); for the
 // thread's elements this is a per-thread base plus j >> (L+1-LO).
-template <bool INV, int CB, int L, int LO, bool LAZY>
+// ONE (the pass holds the transform's top level, so tile_id is 0 in every
+// work-group): butterflies in block 0 (twiddle 1) skip the product.  In the
+// top window (LO = CB-4, every thread's t >> LO = 0) their block is 0 when
+// j >> (L+1-LO) = 0, known at compile time.
+template <bool INV, int CB, int L, int LO, bool LAZY, bool ONE = false>
 __device__ __forceinline__ void level_ct(const Prime& P, u64 (&v)[16], u32 t, u64 tile_id, const u64* __restrict__ tw) {
     static_assert(L >= LO && L <= LO + 3, "level outside the register window");
     constexpr int D = 1 << (L - LO);
@@ -87,11 +91,18 @@ __device__ __forceinline__ void level_ct(const Prime& P, u64 (&v)[16], u32 t, u6
     CFor<0, 16>::run([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         if constexpr ((j & D) == 0) {
-            const u64 w = twb[j >> (L + 1 - LO)];
-            if constexpr (INV && LAZY) gs_bfly_lazy(P, v[j], v[j + D], w);
-            else if constexpr (INV) gs_bfly(P, v[j], v[j + D], w);
-            else if constexpr (LAZY) ct_bfly_lazy(P, v[j], v[j + D], w);
-            else ct_bfly(P, v[j], v[j + D], w);
+            if constexpr (ONE && LO == CB - 4 && (j >> (L + 1 - LO)) == 0) {
+                if constexpr (INV && LAZY) gs_bfly_lazy_one(P, v[j], v[j + D]);
+                else if constexpr (INV) gs_bfly_one(P, v[j], v[j + D]);
+                else if constexpr (LAZY) ct_bfly_lazy_one(P, v[j], v[j + D]);
+                else ct_bfly_one(P, v[j], v[j + D]);
+            } else {
+                const u64 w = twb[j >> (L + 1 - LO)];
+                if constexpr (INV && LAZY) gs_bfly_lazy(P, v[j], v[j + D], w);
+                else if constexpr (INV) gs_bfly(P, v[j], v[j + D], w);
+                else if constexpr (LAZY) ct_bfly_lazy(P, v[j], v[j + D], w);
+                else ct_bfly(P, v[j], v[j + D], w);
+            }
         }
     });
 }
@@ -112,24 +123,24 @@ __host__ __device__ constexpr int inv_window(int l, int l_lo, int lo0, int cb) {
     return lo;
 }
 
-template <int CB, int L, int L_LO, int L_HI, int LO0, int LO_MIN, bool LAZY, class LT>
+template <int CB, int L, int L_LO, int L_HI, int LO0, int LO_MIN, bool LAZY, class LT, bool ONE = false>
 __device__ __forceinline__ void fwd_rec(const Prime& P, u64 (&v)[16], LT* lds, u32 t, u64 tile_id, const u64* __restrict__ tw) {
     if constexpr (L >= L_LO) {
         constexpr int prev = L == L_HI ? LO0 : fwd_window(L + 1, L_HI, LO0, LO_MIN);
         constexpr int cur = fwd_window(L, L_HI, LO0, LO_MIN);
         if constexpr (cur != prev) xchg<CB, prev, cur>(v, lds, t);
-        level_ct<false, CB, L, cur, LAZY>(P, v, t, tile_id, tw);
-        fwd_rec<CB, L - 1, L_LO, L_HI, LO0, LO_MIN, LAZY>(P, v, lds, t, tile_id, tw);
+        level_ct<false, CB, L, cur, LAZY, ONE>(P, v, t, tile_id, tw);
+        fwd_rec<CB, L - 1, L_LO, L_HI, LO0, LO_MIN, LAZY, LT, ONE>(P, v, lds, t, tile_id, tw);
     }
 }
-template <int CB, int L, int L_LO, int L_HI, int LO0, bool LAZY, class LT>
+template <int CB, int L, int L_LO, int L_HI, int LO0, bool LAZY, class LT, bool ONE = false>
 __device__ __forceinline__ void inv_rec(const Prime& P, u64 (&v)[16], LT* lds, u32 t, u64 tile_id, const u64* __restrict__ tw) {
     if constexpr (L <= L_HI) {
         constexpr int prev = L == L_LO ? LO0 : inv_window(L - 1, L_LO, LO0, CB);
         constexpr int cur = inv_window(L, L_LO, LO0, CB);
         if constexpr (cur != prev) xchg<CB, prev, cur>(v, lds, t);
-        level_ct<true, CB, L, cur, LAZY>(P, v, t, tile_id, tw);
-        inv_rec<CB, L + 1, L_LO, L_HI, LO0, LAZY>(P, v, lds, t, tile_id, tw);
+        level_ct<true, CB, L, cur, LAZY, ONE>(P, v, t, tile_id, tw);
+        inv_rec<CB, L + 1, L_LO, L_HI, LO0, LAZY, LT, ONE>(P, v, lds, t, tile_id, tw);
     }
 }
 
@@ -177,8 +188,11 @@ struct TopArgs {
 
 // Top pass over a 4096-element tile of Q = 2^(12-NLEV) consecutive columns
 // by R = 2^NLEV rows of stride d = 2^g_lo (local index y = col + Q row).
-template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY, class LT>
+// TOP: the pass holds the transform's top level (g_lo + NLEV = s), so every
+// tile lies in block 0 of its top level.  The limb-loading pass always does.
+template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY, bool TOP, class LT>
 __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile, LT* lds) {
+    static_assert(INV || TOP == FROM_LIMBS, "the first forward pass loads the limbs");
     const Prime P = make_prime(PRIME_K[0] * (pr == 0) + PRIME_K[1] * (pr == 1) + PRIME_K[2] * (pr == 2));
     constexpr int CB = 12, Q = 12 - NLEV;
     constexpr u32 QM = (1u << Q) - 1u;
@@ -222,12 +236,12 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
         static_assert(LO_IO == CB - 4, "the top level's partner is register j + 8");
         if (top_copy) {
             CFor<0, 8>::run([&](auto jc) { v[decltype(jc)::value + 8] = v[decltype(jc)::value]; });
-            fwd_rec<CB, CB - 2, Q, CB - 1, LO_IO, Q, LAZY>(P, v, lds, t, blk, tw);
+            fwd_rec<CB, CB - 2, Q, CB - 1, LO_IO, Q, LAZY, LT, TOP>(P, v, lds, t, blk, tw);
         } else {
-            fwd_rec<CB, CB - 1, Q, CB - 1, LO_IO, Q, LAZY>(P, v, lds, t, blk, tw);
+            fwd_rec<CB, CB - 1, Q, CB - 1, LO_IO, Q, LAZY, LT, TOP>(P, v, lds, t, blk, tw);
         }
     } else {
-        inv_rec<CB, Q, Q, CB - 1, LO_INV0, LAZY>(P, v, lds, t, blk, tw);
+        inv_rec<CB, Q, Q, CB - 1, LO_INV0, LAZY, LT, TOP>(P, v, lds, t, blk, tw);
     }
     if constexpr (LO_ST != LO_END) xchg<CB, LO_END, LO_ST>(v, lds, t);
     {

@@ -8,10 +8,12 @@
 
 namespace nnh {
 
-template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY>
-__global__ __launch_bounds__(256) void k_top_pass(TopArgs A, u32 pr0) {
+// The limb-loading pass asks for 6 waves per SIMD (80 VGPRs; it needs 81
+// unconstrained, which would allow 5).
+template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY, bool TOP>
+__global__ __launch_bounds__(256, FROM_LIMBS ? 6 : 1) void k_top_pass(TopArgs A, u32 pr0) {
     __shared__ u32 lds[4096 + 256];  // 32-bit transposes: half the LDS, more work-groups per CU
-    top_pass_body<INV, FROM_LIMBS, NLEV, LAZY>(A, pr0 + blockIdx.y, blockIdx.x, lds);
+    top_pass_body<INV, FROM_LIMBS, NLEV, LAZY, TOP>(A, pr0 + blockIdx.y, blockIdx.x, lds);
 }
 
 // Batched: one work-group per (job, tile) item of a bucket of equal
@@ -20,7 +22,7 @@ __global__ __launch_bounds__(256) void k_top_pass(TopArgs A, u32 pr0) {
 // grid-stride loop over items kept its loop state live and cost ~60 VGPRs,
 // one wave per SIMD); the per-prime pointers are resolved before the body,
 // so nothing is indexed by the prime at run time.
-template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY>
+template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY, bool TOP>
 __global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__ jobs, u32 njobs, int operand,
                                                         TopBatchParams P, u32 pr0) {
     __shared__ u32 lds[4096 + 256];  // 32-bit transposes: half the LDS, more work-groups per CU
@@ -42,32 +44,32 @@ __global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__
     A.an = operand ? J.bn : J.an;
     A.g_lo = P.g_lo;
     A.nlev = P.nlev;
-    top_pass_body<INV, FROM_LIMBS, NLEV, LAZY>(A, pr, tile, lds);
+    top_pass_body<INV, FROM_LIMBS, NLEV, LAZY, TOP>(A, pr, tile, lds);
 }
 
 // Host dispatch on the pass depth (1..8 levels), known per launch.
-template <bool INV, bool FL, bool LAZY>
+template <bool INV, bool FL, bool LAZY, bool TOP>
 static void top_single(const TopArgs& A, u32 nlev, dim3 grid, u32 pr0, hipStream_t st) {
     switch (nlev) {
 #define NNH_TP(n) \
-    case n: hipLaunchKernelGGL((k_top_pass<INV, FL, n, LAZY>), grid, dim3(256), 0, st, A, pr0); break;
+    case n: hipLaunchKernelGGL((k_top_pass<INV, FL, n, LAZY, TOP>), grid, dim3(256), 0, st, A, pr0); break;
         NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
-        default: hipLaunchKernelGGL((k_top_pass<INV, FL, 8, LAZY>), grid, dim3(256), 0, st, A, pr0); break;
+        default: hipLaunchKernelGGL((k_top_pass<INV, FL, 8, LAZY, TOP>), grid, dim3(256), 0, st, A, pr0); break;
 #undef NNH_TP
     }
 }
-template <bool INV, bool FL, bool LAZY>
+template <bool INV, bool FL, bool LAZY, bool TOP>
 static void top_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, dim3 grid, u32 pr0,
                       hipStream_t st) {
     switch (P.nlev) {
-#define NNH_TP(n)                                                                                     \
-    case n:                                                                                           \
-        hipLaunchKernelGGL((k_top_pass_batch<INV, FL, n, LAZY>), grid, dim3(256), 0, st, jobs, njobs, \
-                           operand, P, pr0);                                                          \
+#define NNH_TP(n)                                                                                          \
+    case n:                                                                                                \
+        hipLaunchKernelGGL((k_top_pass_batch<INV, FL, n, LAZY, TOP>), grid, dim3(256), 0, st, jobs, njobs, \
+                           operand, P, pr0);                                                               \
         break;
         NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
-        default: hipLaunchKernelGGL((k_top_pass_batch<INV, FL, 8, LAZY>), grid, dim3(256), 0, st, jobs, njobs,
-                                    operand, P, pr0);
+        default: hipLaunchKernelGGL((k_top_pass_batch<INV, FL, 8, LAZY, TOP>), grid, dim3(256), 0, st, jobs,
+                                    njobs, operand, P, pr0);
             break;
 #undef NNH_TP
     }
@@ -87,9 +89,11 @@ static void top_pass_primes(const TopPassDesc& D, hipStream_t st) {
     A.nlev = D.nlev;
     const u32 pr0 = LAZY ? 0 : 2, npr = LAZY ? 2 : 1;
     dim3 grid((unsigned)(D.L >> 12), npr);
-    if (D.inverse) top_single<true, false, LAZY>(A, D.nlev, grid, pr0, st);
-    else if (D.limbs) top_single<false, true, LAZY>(A, D.nlev, grid, pr0, st);
-    else top_single<false, false, LAZY>(A, D.nlev, grid, pr0, st);
+    const bool top = (1ull << (D.g_lo + D.nlev)) == D.L;  // the pass holds the top level
+    if (D.inverse && top) top_single<true, false, LAZY, true>(A, D.nlev, grid, pr0, st);
+    else if (D.inverse) top_single<true, false, LAZY, false>(A, D.nlev, grid, pr0, st);
+    else if (D.limbs) top_single<false, true, LAZY, true>(A, D.nlev, grid, pr0, st);
+    else top_single<false, false, LAZY, false>(A, D.nlev, grid, pr0, st);
 }
 
 template <bool LAZY>
@@ -97,9 +101,11 @@ static void top_pass_batch_primes(const BJob* jobs, u32 njobs, int operand, cons
                                   bool from_limbs, hipStream_t st) {
     const u32 pr0 = LAZY ? 0 : 2, npr = LAZY ? 2 : 1;
     dim3 grid((unsigned)((u64)njobs * (P.L >> 12)), npr);
-    if (inverse) top_batch<true, false, LAZY>(jobs, njobs, operand, P, grid, pr0, st);
-    else if (from_limbs) top_batch<false, true, LAZY>(jobs, njobs, operand, P, grid, pr0, st);
-    else top_batch<false, false, LAZY>(jobs, njobs, operand, P, grid, pr0, st);
+    const bool top = (1ull << (P.g_lo + P.nlev)) == P.L;  // the pass holds the top level
+    if (inverse && top) top_batch<true, false, LAZY, true>(jobs, njobs, operand, P, grid, pr0, st);
+    else if (inverse) top_batch<true, false, LAZY, false>(jobs, njobs, operand, P, grid, pr0, st);
+    else if (from_limbs) top_batch<false, true, LAZY, true>(jobs, njobs, operand, P, grid, pr0, st);
+    else top_batch<false, false, LAZY, false>(jobs, njobs, operand, P, grid, pr0, st);
 }
 
 }  // namespace nnh

@@ -45,3 +45,19 @@ def test_missing_library_fails_loudly(monkeypatch):
         assert "not built" in str(e)
     else:
         raise AssertionError("loading a missing library must raise")
+
+
+def test_transform_plan_splits_every_level():
+    """Every transform level runs exactly once: chunk_bits - 1 in the fused
+    chunk kernel, the rest in top passes of at most 8 levels (the base
+    multiply takes the last level); the 2^25-word product uses 2^10 chunks
+    and passes of 8 and 7 levels."""
+    import natural_number_amd as nn
+
+    assert nn.transform_plan(25) == (10, [8, 7])
+    for s in range(2, 31):
+        cb, passes = nn.transform_plan(s)
+        assert cb + sum(passes) == s
+        assert all(1 <= p <= 8 for p in passes)
+        assert cb <= 12 and (cb == s or cb >= 10)
+        assert len(passes) == (0 if s <= 12 else (s - 12 + 7) // 8)
