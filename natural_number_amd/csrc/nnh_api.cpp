@@ -76,6 +76,18 @@ struct Device {
     // chunk kernel and the launch tails overlap.
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // Batches spread their buckets (independent until the shared CRT) over
+    // the caller's (or capturing) stream and nwork worker streams (work[0] is
+    // stream2; NNH_BATCH_STREAMS=0..4 sets nwork): one bucket's p0/p1 chain
+    // on one lane, its p2 chain on the next, forked from and joined back into
+    // the caller's stream; under graph capture they become parallel branches.
+    // 4 workers: -3 % on the 4096-product mix.  (Four freshly created worker
+    // streams instead of three plus stream2 slowed the two-stream product by
+    // 3 % with the default 4 hardware queues.)
+    static constexpr int NWORK = 4;
+    int nwork = NWORK;
+    hipStream_t work[NWORK]{};
+    hipEvent_t ev_bfork = nullptr, ev_bjoin[NWORK]{};
     bool two_streams = true;
     RootBases rb{};
     CrtConsts crt{};
@@ -227,6 +239,18 @@ static int device_init(int dev) {
     if (!g_dev.stream2) HIPCHK(hipStreamCreateWithFlags(&g_dev.stream2, hipStreamNonBlocking));
     if (!g_dev.ev_fork) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_fork, hipEventDisableTiming));
     if (!g_dev.ev_join) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_join, hipEventDisableTiming));
+    {
+        const char* e = std::getenv("NNH_BATCH_STREAMS");
+        if (e) g_dev.nwork = std::max(0, std::min(Device::NWORK, std::atoi(e)));
+    }
+    for (int w = 0; w < g_dev.nwork; w++) {
+        if (!g_dev.work[w]) {
+            if (w == 0) g_dev.work[w] = g_dev.stream2;
+            else HIPCHK(hipStreamCreateWithFlags(&g_dev.work[w], hipStreamNonBlocking));
+        }
+        if (!g_dev.ev_bjoin[w]) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_bjoin[w], hipEventDisableTiming));
+    }
+    if (!g_dev.ev_bfork) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_bfork, hipEventDisableTiming));
     {
         const char* e = std::getenv("NNH_ONE_STREAM");
         g_dev.two_streams = !(e && e[0] == '1');
@@ -681,9 +705,21 @@ static void batch_free(nnh_batch* B) {
 }
 
 // Enqueue every launch of the batch on st.
-static void batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
+static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
+    const bool fan = D.two_streams && D.nwork > 0 && B->buckets.size() > 0;
+    if (fan) {
+        HIPCHK(hipEventRecord(D.ev_bfork, st));
+        for (int w = 0; w < D.nwork; w++) HIPCHK(hipStreamWaitEvent(D.work[w], D.ev_bfork, 0));
+    }
+    // lanes: st and the workers
+    int lane = 0;
+    auto next_stream = [&]() {
+        const int l = lane++ % (D.nwork + 1);
+        return (fan && l) ? D.work[l - 1] : st;
+    };
     if (B->n_school) launch_school_batch(B->d_jobs, B->d_sj, B->n_school, B->ncols, st);
     for (const Bucket& bk : B->buckets) {
+        const hipStream_t sa = next_stream(), sb = next_stream();
         Plan pl = make_plan(D, bk.s);
         const BJob* jobs = B->d_jobs + bk.first;
         TopBatchParams tp{};
@@ -696,7 +732,7 @@ static void batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
                 for (int i = 0; i < 3; i++) tp.tw[i] = D.twf[i];
                 tp.g_lo = (u32)passes[k].first;
                 tp.nlev = (u32)passes[k].second;
-                launch_top_pass_batch(jobs, bk.count, op, tp, false, k == 0, st);
+                launch_top_pass_batch(jobs, bk.count, op, tp, false, k == 0, sa, sb);
             }
         MidBatchParams mp{};
         for (int i = 0; i < 3; i++) {
@@ -705,16 +741,23 @@ static void batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
             mp.alpha_m[i] = pl.alpha_m[i];
         }
         mp.L = pl.L;
-        if (bk.own_count) launch_mid_batch(B->d_jobs + bk.own_first, bk.own_count, mp, pl.cb, pl.ntop == 0, true, st);
-        launch_mid_batch(jobs, bk.count, mp, pl.cb, pl.ntop == 0, false, st);
+        if (bk.own_count)
+            launch_mid_batch(B->d_jobs + bk.own_first, bk.own_count, mp, pl.cb, pl.ntop == 0, true, sa, sb);
+        launch_mid_batch(jobs, bk.count, mp, pl.cb, pl.ntop == 0, false, sa, sb);
         for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
             for (int i = 0; i < 3; i++) tp.tw[i] = D.twi[i];
             tp.g_lo = (u32)it->first;
             tp.nlev = (u32)it->second;
-            launch_top_pass_batch(jobs, bk.count, 0, tp, true, false, st);
+            launch_top_pass_batch(jobs, bk.count, 0, tp, true, false, sa, sb);
         }
     }
+    if (fan)
+        for (int w = 0; w < D.nwork; w++) {
+            HIPCHK(hipEventRecord(D.ev_bjoin[w], D.work[w]));
+            HIPCHK(hipStreamWaitEvent(st, D.ev_bjoin[w], 0));
+        }
     launch_crt_batch(B->d_jobs, B->n_req_jobs, B->ntiles, B->d_tiles, B->d_tiles + B->ntiles, D.crt, st);
+    return NNH_OK;
 }
 
 static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch** out) {
@@ -872,8 +915,9 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
     if (flags & 1) {
         HIPCHK(hipStreamCreateWithFlags(&B->cap_stream, hipStreamNonBlocking));
         HIPCHK(hipStreamBeginCapture(B->cap_stream, hipStreamCaptureModeThreadLocal));
-        batch_enqueue(D, B, B->cap_stream);
+        const int erc = batch_enqueue(D, B, B->cap_stream);
         HIPCHK(hipStreamEndCapture(B->cap_stream, &B->graph));
+        if (erc) return erc;
         HIPCHK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
     }
     *out = B;
@@ -918,8 +962,13 @@ int dev_batch(const IReq* reqs, u64 n, hipStream_t st) {
     nnh_batch* B = nullptr;
     int rc = batch_create(g_dev, reqs, n, 2, &B);
     if (rc) return rc;
-    batch_enqueue(g_dev, B, st);
+    rc = batch_enqueue(g_dev, B, st);
     hipError_t e = hipGetLastError();
+    if (rc) {
+        (void)hipStreamSynchronize(st);
+        batch_free(B);
+        return rc;
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     batch_free(B);
     if (e != hipSuccess) return fail(NNH_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
@@ -1219,7 +1268,7 @@ int nnh_batch_run(nnh_batch* b, void* stream) {
     std::lock_guard<std::mutex> g(g_dev.mu);
     hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
     if (b->exec) HIPCHK(hipGraphLaunch(b->exec, st));
-    else batch_enqueue(g_dev, b, st);
+    else if (int rc = batch_enqueue(g_dev, b, st)) return rc;
     HIPCHK(hipGetLastError());
     return NNH_OK;
 }

@@ -99,11 +99,12 @@ struct MidBatchParams {
     u64 L;
 };
 
+// (The p0/p1 kernels go to st, the p2 kernels to st2, st when null.)
 void launch_top_pass_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
-                           bool from_limbs, hipStream_t st);
+                           bool from_limbs, hipStream_t st, hipStream_t st2 = nullptr);
 // fwd_only: the jobs are forward-only entries that store shared B transforms.
 void launch_mid_batch(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs, bool fwd_only,
-                      hipStream_t st);
+                      hipStream_t st, hipStream_t st2 = nullptr);
 void launch_school_batch(const BJob* jobs, const u32* sj, u32 nsj, u64 ncols, hipStream_t st);
 void launch_crt_batch(const BJob* jobs, u32 njobs, u64 ntiles, u8* tile_fn, u8* tile_cin, const CrtConsts& C,
                       hipStream_t st);

@@ -14,9 +14,9 @@ void launch_mid(const MidDesc& D, hipStream_t st, hipStream_t st2) {
 }
 
 void launch_mid_batch(const BJob* jobs, u32 njobs, const MidBatchParams& P, int cb, bool from_limbs, bool fwd_only,
-                      hipStream_t st) {
+                      hipStream_t st, hipStream_t st2) {
     launch_mid_batch_part(jobs, njobs, P, cb, from_limbs, fwd_only, true, st);
-    mid_batch_primes<false>(jobs, njobs, P, cb, from_limbs, fwd_only, st);
+    mid_batch_primes<false>(jobs, njobs, P, cb, from_limbs, fwd_only, st2 ? st2 : st);
 }
 
 }  // namespace nnh

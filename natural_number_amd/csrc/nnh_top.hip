@@ -14,9 +14,9 @@ void launch_top_pass(const TopPassDesc& D, hipStream_t st, hipStream_t st2) {
 }
 
 void launch_top_pass_batch(const BJob* jobs, u32 njobs, int operand, const TopBatchParams& P, bool inverse,
-                           bool from_limbs, hipStream_t st) {
+                           bool from_limbs, hipStream_t st, hipStream_t st2) {
     launch_top_pass_batch_part(jobs, njobs, operand, P, inverse, from_limbs, true, st);
-    top_pass_batch_primes<false>(jobs, njobs, operand, P, inverse, from_limbs, st);
+    top_pass_batch_primes<false>(jobs, njobs, operand, P, inverse, from_limbs, st2 ? st2 : st);
 }
 
 }  // namespace nnh
