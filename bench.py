@@ -30,7 +30,7 @@ HBM_PEAK = 8.0e12
 # HBM bytes per launch of the fused chunk kernel at this workload: rocprofv3
 # FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the counter's reading
 # for a plain copy of known size (profiles/ntt_product_2e30x2e30_pmc.md).
-MID_FUSED_TRAFFIC = 3.501e9  # HBM bytes per product of both chunk-kernel launches, rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated)
+MID_FUSED_TRAFFIC = 3.545e9  # HBM bytes per product of both chunk-kernel launches, rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated)
 
 
 def log(*a):
