@@ -54,5 +54,21 @@ def build(verbose=False):
     return LIB
 
 
+def build_arith_check(verbose=False):
+    """The device arithmetic check run by tests/test_gpu_arith.py (test
+    infrastructure, built beside its source)."""
+    src_dir = os.path.join(os.path.dirname(HERE), "tests", "cpp")
+    exe = os.path.join(src_dir, "arith_dev_check")
+    cmd = ["hipcc", "arith_dev_check.hip", "-o", exe + ".tmp", "-O3", "-std=c++20", f"--offload-arch={ARCH}"] + HIP_FLAGS
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=src_dir, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -74,6 +74,25 @@ static int check_prime(std::mt19937_64& rng, int iters, int pi) {
             base_mul4<true>(P, l0, l1, l2, l3, in[4], in[5], in[6], in[7], wm);
             base_mul4<false>(P, s0, s1, s2, s3, red[4], red[5], red[6], red[7], wm);
             if (l0 != s0 || l1 != s1 || l2 != s2 || l3 != s3) bad++;
+            // twiddle-1 butterflies (block 0) and the 2p conditional subtraction
+            u64 ox = lx, oy = ly;
+            ct_bfly_lazy_one(P, ox, oy);
+            if (ox >= 4 * p || oy >= 4 * p || ox % p != (lx % p + ly % p) % p || oy % p != (lx % p + p - ly % p) % p)
+                bad++;
+            u64 qx = ix, qy = iy;
+            gs_bfly_lazy_one(P, qx, qy);
+            if (qx >= 2 * p || qy >= 2 * p || qx % p != (ix % p + iy % p) % p || qy % p != (ix % p + p - iy % p) % p)
+                bad++;
+            if (csub2(P, lx) >= 2 * p || csub2(P, lx) % p != lx % p) bad++;
+        }
+        {
+            u64 ox = bx, oy = by;
+            ct_bfly_one(P, ox, oy);
+            if (ox >= 2 * p || oy >= 2 * p || ox % p != (bx % p + by % p) % p || oy % p != (bx % p + p - by % p) % p)
+                bad++;
+            u64 qx = gx, qy = gy;
+            gs_bfly_one(P, qx, qy);
+            if (qx != (gx + gy) % p || qy != (gx + p - gy) % p) bad++;
         }
     }
     // reduction edge cases: zero words, all-ones words, top byte patterns of the quotient
