@@ -233,7 +233,9 @@ def cost_model_run(n_side, n_held):
     model = cm.CostModel(seed=7)
     t1 = time.perf_counter()
     loss0 = model.train(X, T, M, steps=1)
-    loss = model.train(X, T, M, steps=4000, lr=3e-3)
+    # Adam, full batch: 4000 steps at each of three decreasing rates
+    for lr in (3e-3, 1e-3, 3e-4):
+        loss = model.train(X, T, M, steps=4000, lr=lr)
     t_train = time.perf_counter() - t1
     reqs = log_uniform_requests(n_held, 0x4E4E2E1972800005)
     log(f"cost model: evaluating {len(reqs)} held-out requests")

@@ -102,8 +102,10 @@ def grid(n_side=20, lo_bits=10, hi_bits=30, counts=(1, 8, 64)):
     return pts
 
 
-def collect(bench, points, log=None):
-    """Measured log2(ms per product) for every feasible arm of every point."""
+def collect(bench, points, log=None, repeats=2):
+    """Measured log2(ms per product) for every feasible arm of every point:
+    the fastest of `repeats` timings, so that one disturbed measurement
+    does not bend the fitted landscape around its grid point."""
     X = np.zeros((len(points), N_IN), dtype=np.float32)
     T = np.zeros((len(points), N_ARMS), dtype=np.float32)
     M = np.zeros((len(points), N_ARMS), dtype=np.float32)
@@ -113,7 +115,7 @@ def collect(bench, points, log=None):
         for arm in range(N_ARMS):
             if f[arm]:
                 algo, param = arm_call(arm, an, bn)
-                ms = bench.time(algo, param, an, bn, count=c)
+                ms = min(bench.time(algo, param, an, bn, count=c) for _ in range(repeats))
                 T[k, arm] = math.log2(max(ms, 1e-6))
                 M[k, arm] = 1.0
         if log and k % 50 == 0:
