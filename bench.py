@@ -142,15 +142,18 @@ def e_workload(nn, with_cpu):
     for name in ("e_k21_D10000000", "e_k24_D100000000"):
         g = golden[name]
         log(f"e: {g['digits']} digits on the GPU")
+        # timed like the oracle below: the library call writing into a
+        # caller's buffer, without the Python-side buffer set-up and decode
+        buf = ctypes.create_string_buffer(g["digits"] + 64)
         t = time.perf_counter()
-        s, stages = nn.e_digits(g["k"], g["digits"])
+        n, stages = nn.e_digits_into(buf, g["k"], g["digits"])
         wall = time.perf_counter() - t
         res[name] = {
             "k": g["k"], "digits": g["digits"], "gpu_wall_s": round(wall, 4),
             "stages_ms": {k: round(v, 3) for k, v in stages.items()},
-            "digits_match_reference": hashlib.sha256(s.encode()).hexdigest() == g["sha256"],
+            "digits_match_reference": hashlib.sha256(buf.raw[:n]).hexdigest() == g["sha256"],
         }
-        del s
+        del buf
     if with_cpu:
         g = golden["e_k21_D10000000"]
         log("e: oracle on 10^7 digits (1 thread)")
@@ -181,12 +184,13 @@ def drivers_workload(nn, with_cpu):
     for name in ("pi_k17_D1000000_acc60000", "pi_k20_D10000000_acc600000"):
         g = gp[name]
         log(f"pi: {g['digits']} digits on the GPU")
+        buf = ctypes.create_string_buffer(g["digits"] + 64)  # timed like the oracle: the library call only
         t = time.perf_counter()
-        s, stages = nn.pi_digits(g["k"], g["digits"], g["acc"])
+        n, stages = nn.pi_digits_into(buf, g["k"], g["digits"], g["acc"])
         wall = time.perf_counter() - t
         res[name] = {"gpu_wall_s": round(wall, 4), "stages_ms": {k: round(v, 3) for k, v in stages.items()},
-                     "digits_match_reference": hashlib.sha256(s.encode()).hexdigest() == g["sha256"]}
-        del s
+                     "digits_match_reference": hashlib.sha256(buf.raw[:n]).hexdigest() == g["sha256"]}
+        del buf
     if with_cpu:
         g = gp["pi_k17_D1000000_acc60000"]
         log("pi: oracle on 10^6 digits (1 thread)")

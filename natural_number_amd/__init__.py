@@ -394,28 +394,40 @@ def calc_pi(k):
 PI_STAGES = ["calc_pi", "sqrt_10005", "products", "div", "to_dec"]
 
 
-def pi_digits(k, digits, acc):
-    """The reference's pi recipe (test.cpp:22-32): (digit string, {stage: ms})."""
-    cap = digits + 64
-    buf = ctypes.create_string_buffer(cap)
+def pi_digits_into(buf, k, digits, acc):
+    """The reference's pi recipe (test.cpp:22-32) into a ctypes char buffer:
+    (digit count, {stage: ms})."""
     n = ctypes.c_uint64()
     ms = (ctypes.c_double * len(PI_STAGES))()
-    _check(lib().nnh_pi_digits(buf, cap, ctypes.byref(n), k, digits, acc, ms))
-    return buf.raw[:n.value].decode(), dict(zip(PI_STAGES, list(ms)))
+    _check(lib().nnh_pi_digits(buf, len(buf), ctypes.byref(n), k, digits, acc, ms))
+    return n.value, dict(zip(PI_STAGES, list(ms)))
+
+
+def pi_digits(k, digits, acc):
+    """The reference's pi recipe (test.cpp:22-32): (digit string, {stage: ms})."""
+    buf = ctypes.create_string_buffer(digits + 64)
+    n, st = pi_digits_into(buf, k, digits, acc)
+    return buf.raw[:n].decode(), st
 
 
 E_STAGES = ["calc_e", "pow", "mul", "div", "to_dec"]
 
 
+def e_digits_into(buf, k, digits):
+    """The reference's e recipe (test.cpp:12-19) into a ctypes char buffer:
+    (digit count, {stage: ms})."""
+    n = ctypes.c_uint64()
+    ms = (ctypes.c_double * len(E_STAGES))()
+    _check(lib().nnh_e_digits(buf, len(buf), ctypes.byref(n), k, digits, ms))
+    return n.value, dict(zip(E_STAGES, list(ms)))
+
+
 def e_digits(k, digits):
     """The reference's e recipe (test.cpp:12-19): returns (digit string,
     {stage: ms})."""
-    cap = digits + 64
-    buf = ctypes.create_string_buffer(cap)
-    n = ctypes.c_uint64()
-    ms = (ctypes.c_double * len(E_STAGES))()
-    _check(lib().nnh_e_digits(buf, cap, ctypes.byref(n), k, digits, ms))
-    return buf.raw[:n.value].decode(), dict(zip(E_STAGES, list(ms)))
+    buf = ctypes.create_string_buffer(digits + 64)
+    n, st = e_digits_into(buf, k, digits)
+    return buf.raw[:n].decode(), st
 
 
 def threshold_words():
