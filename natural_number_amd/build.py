@@ -70,5 +70,23 @@ def build_arith_check(verbose=False):
     return exe
 
 
+def build_natural_check(verbose=False):
+    """The C++ class check run by tests/test_gpu_natural_cpp.py (host code
+    over include/nnh_natural.hpp, linked to the in-tree libnnh.so)."""
+    root = os.path.dirname(HERE)
+    src_dir = os.path.join(root, "tests", "cpp")
+    exe = os.path.join(src_dir, "natural_check")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "natural_check.cpp", "-o", exe + ".tmp",
+           "-I", os.path.join(root, "include"), "-L", HERE, "-lnnh", "-Wl,-rpath,$ORIGIN/../../natural_number_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=src_dir, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
