@@ -44,11 +44,14 @@ int nnh_init(int device);
 int nnh_shutdown(void);
 
 /* dst[0 .. an+bn) = a * b for host buffers.  dst is not normalised: the
- * caller trims top zero words (natural::std, natural.h:82-93).
+ * caller trims top zero words (natural::std, natural.h:82-93).  dst may
+ * overlap a or b (a *= a, natural.h:357-362): both operands are staged in
+ * device memory before any result word is written back.
  * Reference: natural::_mul (natural.h:444-450). */
 int nnh_mul(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn);
 
-/* dst[0 .. 2an) = a^2.  Reference: natural::_sqr (natural.h:529-535). */
+/* dst[0 .. 2an) = a^2; dst may overlap a.  Reference: natural::_sqr
+ * (natural.h:529-535). */
 int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an);
 
 /* Explicit algorithm arms (no counterpart in the reference, whose choice is
@@ -97,7 +100,8 @@ int nnh_mlp_choose(nnh_mlp* m, const float* x, const float* feasible, uint64_t n
 
 /* Device-pointer variants: operands and result already in HBM, work is
  * queued on `stream` (a hipStream_t, NULL = the library's stream) and the
- * call returns without synchronising. */
+ * call returns without synchronising.  d_dst must not overlap d_a or d_b
+ * (the split path writes its first half before loading the second). */
 int nnh_mul_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an,
                          const uint64_t* d_b, uint64_t bn, void* stream);
 int nnh_sqr_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, void* stream);
