@@ -67,6 +67,7 @@ class natural {
     friend natural pow(const natural& b, uint64_t e) {
         ready();
         if (e == 0) return natural(1);
+        if (e > UINT64_MAX / b.bits()) throw std::length_error("natural: pow result too large");
         const uint64_t cap = (b.bits() * e) / 64 + 2;
         std::vector<uint64_t> r(cap);
         uint64_t n = 0;

@@ -61,3 +61,17 @@ def test_transform_plan_splits_every_level():
         assert all(1 <= p <= 8 for p in passes)
         assert cb <= 12 and (cb == s or cb >= 10)
         assert len(passes) == (0 if s <= 12 else (s - 12 + 7) // 8)
+
+
+def test_cpp_class_header_compiles(tmp_path):
+    """include/nnh_natural.hpp compiles on its own (C++17 and C++20, warnings
+    as errors); the GPU behaviour is covered by test_gpu_natural_cpp.py."""
+    import subprocess
+
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "nnh_natural.hpp"\n'
+                   "int main() { nnh::natural a(3), b = a; return cmp(a, b) + (int)(a + b).size(); }\n")
+    for std in ("c++17", "c++20"):
+        r = subprocess.run(["g++", f"-std={std}", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                            "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
