@@ -68,9 +68,62 @@ def batch_ideal_ms(sizes, r_lazy, r_strict):
     return t * 1e3
 
 
-def batch_mix(nn, stream, valu, reps=5):
+def batch_cpu_baseline(ops, sizes, gpu_out=None, stride=4, max_threads=16):
+    """The oracle on the batch mix: one thread over every `stride`-th request
+    (a bounded sample of the same size distribution, ~10 s), and the host's
+    cores (at most `max_threads` threads) over the whole mix, each product
+    compared word for word with the GPU's (gpu_out: the batch's outputs,
+    concatenated in request order).  The oracle's
+    twiddle tables are global and grown on demand, like the reference's
+    (ntt_info.h:141-145), so they are grown to the mix's largest scale
+    first; the threads then only read them."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    p = ctypes.POINTER(ctypes.c_uint64)
+    lib.nno_mul.restype = ctypes.c_uint64
+    lib.nno_mul.argtypes = [p, p, ctypes.c_uint64, p, ctypes.c_uint64]
+    lib.nno_twiddles.restype = None
+    lib.nno_twiddles.argtypes = [p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    smax = max((an + bn - 2).bit_length() for an, bn in sizes)
+    tw = np.empty(1 << (smax - 1), dtype=np.uint64)
+    lib.nno_twiddles(tw.ctypes.data_as(p), 0, smax, 0)
+
+    offs = np.concatenate([[0], np.cumsum([a.size + b.size for a, b in ops])])
+
+    def one(i):
+        a, b = ops[i]
+        out = np.empty(a.size + b.size, dtype=np.uint64)
+        lib.nno_mul(out.ctypes.data_as(p), a.ctypes.data_as(p), a.size, b.ctypes.data_as(p), b.size)
+        return gpu_out is None or np.array_equal(out, gpu_out[offs[i]:offs[i + 1]])
+
+    sample = range(0, len(ops), stride)
+    log(f"batch mix: cpu baseline, 1 thread over {len(sample)} requests")
+    t = time.perf_counter()
+    for i in sample:
+        one(i)
+    t1 = time.perf_counter() - t
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    log(f"batch mix: cpu baseline, {threads} threads over the whole mix")
+    t = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        same = list(ex.map(one, range(len(ops))))
+    tn = time.perf_counter() - t
+    return {
+        "one_thread_products_per_s": len(sample) / t1,
+        "one_thread_sample": f"every {stride}th request of the mix ({len(sample)} products), {t1:.2f} s",
+        "all_cores_products_per_s": len(ops) / tn,
+        "all_cores_threads": threads,
+        "all_cores_sample": f"the whole mix ({len(ops)} products), {tn:.2f} s",
+        "gpu_products_identical": None if gpu_out is None else int(sum(same)),
+        "kind": "port",
+    }
+
+
+def batch_mix(nn, stream, valu, reps=5, with_cpu=True):
     """Config 3: 4096 uneven products; batched (bucketed kernels,
-    one hipGraph) against one launch sequence per product on the same stream."""
+    one hipGraph) against one launch sequence per product on the same stream;
+    with_cpu: the oracle's products/s on the same mix beside them."""
     from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
 
     log("batch mix: generating operands")
@@ -96,7 +149,8 @@ def batch_mix(nn, stream, valu, reps=5):
     for r in reqs_s:
         nn.mul_device_async(*r, sp)
     nn.synchronize()
-    exact = bool(np.array_equal(out_b.download(), out_s.download()))
+    host_b = out_b.download()
+    exact = bool(np.array_equal(host_b, out_s.download()))
     e0, e1 = nn.Event(), nn.Event()
     nn.synchronize()
     e0.record(stream)
@@ -126,6 +180,7 @@ def batch_mix(nn, stream, valu, reps=5):
         "speedup": ts / tb,
         "bit_exact_batched_vs_single": exact,
         "valu_ceiling": {"ideal_ms": round(ideal, 3), "frac": round(ideal / tb, 3)},
+        "cpu_baseline": batch_cpu_baseline(ops, sizes, host_b) if with_cpu else None,
     }
 
 
@@ -408,7 +463,7 @@ def main():
 
     valu = valu_ceiling(nn, kernels, args.steps)
 
-    batch = None if args.no_batch else batch_mix(nn, stream, valu)
+    batch = None if args.no_batch else batch_mix(nn, stream, valu, with_cpu=not args.no_cpu)
     e_res = None if args.no_e else e_workload(nn, not args.no_cpu)
     drv = None if args.no_drivers else drivers_workload(nn, not args.no_cpu)
 
