@@ -68,6 +68,76 @@ def batch_ideal_ms(sizes, r_lazy, r_strict):
     return t * 1e3
 
 
+def product_alg_bytes(s1, s2):
+    """Algorithmic HBM bytes of one product: 16 (s1 + s2) + 144 2^s with
+    s = bit_length(s1 + s2 - 2) (limbs in and out once; three residue words
+    per coefficient written and read by each of the forward, base-multiply
+    and inverse stages)."""
+    return 16 * (s1 + s2) + 144 * (1 << (s1 + s2 - 2).bit_length())
+
+
+def c2_variants(nn, stream, steps, with_cpu):
+    """The other two plain products at scale: 2^30 x 2^22 bits (2^24 x 2^16
+    words, seeds 5 and 6; the reference's one-level split, natural.h:408-418)
+    checked against its committed digest, and two all-ones 2^28-bit operands
+    (2^22 words each) checked against the closed form (B^n - 1)^2 =
+    B^2n - 2 B^n + 1.  Device time per product over `steps` products with
+    operands resident in HBM; with_cpu: the oracle (one thread) on the same
+    product, compared word for word."""
+    golden = json.load(open(os.path.join(ROOT, "tests", "golden", "products.json")))["c2_2^30x2^22"]
+    n1 = 1 << 22
+    cases = [("2^30x2^22", nn.splitmix64(5, 1 << 24), nn.splitmix64(6, 1 << 16)),
+             ("all_ones_2^28x2^28", np.full(n1, 2**64 - 1, dtype=np.uint64), np.full(n1, 2**64 - 1, dtype=np.uint64))]
+    res = {}
+    for name, a_h, b_h in cases:
+        log(f"c2 {name}")
+        an, bn = a_h.size, b_h.size
+        a, b, out = nn.DeviceWords(an, a_h), nn.DeviceWords(bn, b_h), nn.DeviceWords(an + bn)
+        nn.mul_device_async(out.ptr, a.ptr, an, b.ptr, bn, stream.ptr)
+        nn.synchronize()
+        e0, e1 = nn.Event(), nn.Event()
+        e0.record(stream)
+        for _ in range(steps):
+            nn.mul_device_async(out.ptr, a.ptr, an, b.ptr, bn, stream.ptr)
+        e1.record(stream)
+        nn.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        d = out.download()
+        if name.startswith("all_ones"):
+            want = np.full(an + bn, 2**64 - 1, dtype=np.uint64)
+            want[0] = 1
+            want[1:an] = 0
+            want[an] = 2**64 - 2
+            check = {"closed_form": bool(np.array_equal(d, want))}
+        else:
+            check = {"golden_sha256": hashlib.sha256(d.astype("<u8").tobytes()).hexdigest() == golden["sha256"]}
+        r = {"operand_words": [an, bn], "ms": round(ms, 4), "bits_per_s": an * 64 / (ms * 1e-3),
+             "b_alg_gbps": round(product_alg_bytes(an, bn) / (ms * 1e-3) / 1e9, 1),
+             "hbm_roofline_frac": round(product_alg_bytes(an, bn) / (ms * 1e-3) / HBM_PEAK, 4), **check}
+        if name == "2^30x2^22":
+            # the library's chunked arm: B transformed once at a small length,
+            # reused by every piece of A (exact, as every arm is)
+            out2 = nn.DeviceWords(an + bn)
+            nn.mul_algo_device_async(out2.ptr, a.ptr, an, b.ptr, bn, nn.ALGO_NTT_CHUNK, 0, stream.ptr)
+            nn.synchronize()
+            e0.record(stream)
+            for _ in range(steps):
+                nn.mul_algo_device_async(out2.ptr, a.ptr, an, b.ptr, bn, nn.ALGO_NTT_CHUNK, 0, stream.ptr)
+            e1.record(stream)
+            nn.synchronize()
+            r["chunked_arm_ms"] = round(e0.elapsed_ms(e1) / steps, 4)
+            r["chunked_arm_identical"] = bool(np.array_equal(out2.download(), d))
+            out2.free()
+        if with_cpu:
+            ref, tcpu = cpu_baseline(a_h, b_h)
+            r["cpu_oracle_s"] = round(tcpu, 3)
+            r["bit_exact_vs_cpu"] = bool(np.array_equal(ref, d))
+        for x in (a, b, out):
+            x.free()
+        res[name] = r
+    return res
+
+
 def batch_cpu_baseline(ops, sizes, gpu_out=None, stride=4, max_threads=16):
     """The oracle on the batch mix: one thread over every `stride`-th request
     (a bounded sample of the same size distribution, ~10 s), and the host's
@@ -463,6 +533,7 @@ def main():
 
     valu = valu_ceiling(nn, kernels, args.steps)
 
+    c2 = c2_variants(nn, stream, max(3, args.steps // 4), not args.no_cpu)
     batch = None if args.no_batch else batch_mix(nn, stream, valu, with_cpu=not args.no_cpu)
     e_res = None if args.no_e else e_workload(nn, not args.no_cpu)
     drv = None if args.no_drivers else drivers_workload(nn, not args.no_cpu)
@@ -523,6 +594,7 @@ def main():
         "valu_ceiling": valu,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "c2_variants": c2,
         "batch_mix": batch,
         "e_digits": e_res,
         "binary_splitting_drivers": drv,

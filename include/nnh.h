@@ -64,7 +64,8 @@ int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an);
 #define NNH_ALGO_KARATSUBA 2   /* one Karatsuba level over schoolbook columns (toom22mul) */
 #define NNH_ALGO_NTT 3         /* one transform for the whole product */
 #define NNH_ALGO_NTT_SPLIT 4   /* the reference's one-level unbalanced split (_mul_NTT) */
-#define NNH_ALGO_NTT_CHUNK 5   /* larger operand in pieces, shorter operand's transform reused */
+#define NNH_ALGO_NTT_CHUNK 5   /* larger operand in pieces, all pieces one batch sharing the shorter
+                                  operand's transform; synchronises the stream */
 int nnh_mul_algo(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn, int algo, int param);
 int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
                               int algo, int param, void* stream);

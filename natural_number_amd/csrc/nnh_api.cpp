@@ -22,6 +22,7 @@
 #include "../../include/nnh.h"
 #include "nnh_host.hpp"
 #include "nnh_internal.hpp"
+#include "nnh_nat.hpp"
 
 namespace nnh {
 
@@ -580,16 +581,14 @@ static int ntt_direct(Device& D, u64* out, const u64* a, u64 an, const u64* b, u
 }
 
 // The larger operand cut into pieces of P = 2^sp - bn + 1 words, each
-// multiplied through a transform of scale sp that reuses b's transform;
-// piece products are accumulated through the CRT addend.
-static int ntt_chunked(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, int sp, hipStream_t st) {
-    if (an < bn) {
-        std::swap(a, b);
-        std::swap(an, bn);
-    }
-    if (sp <= 0) sp = ntt_scale(bn, bn);
-    sp = std::max(sp, MIN_MID_BITS);
-    if (sp > MAX_SCALE || (1ull << sp) < bn) return fail(NNH_ERR_ARG, "bad piece scale");
+// multiplied by b through a transform of scale sp; piece i's product
+// (P + bn words) starts at word i P.
+//
+// Sequential form: one product per piece reusing b's transform, each piece
+// added through the CRT addend (used when the pieces are so short that the
+// grouped form below would need more than 4 buffers).
+static int ntt_chunked_seq(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, int sp,
+                           hipStream_t st) {
     Plan pl = make_plan(D, sp);
     u64 P = pl.L - bn + 1;
     int rc;
@@ -606,6 +605,44 @@ static int ntt_chunked(Device& D, u64* out, const u64* a, u64 an, const u64* b, 
                 return rc;
         }
     }
+    return NNH_OK;
+}
+
+// Grouped form: all pieces run as one batch whose jobs share b's transform.
+// Pieces G = ceil((P + bn) / P) apart do not overlap, so piece i lands in
+// buffer i mod G (buffer 0 is `out`; all buffers zero outside their
+// pieces) and G - 1 carry-lookahead additions sum the buffers.  The batch
+// synchronises the stream.
+static int ntt_chunked(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, int sp, hipStream_t st) {
+    if (an < bn) {
+        std::swap(a, b);
+        std::swap(an, bn);
+    }
+    if (sp <= 0) sp = ntt_scale(bn, bn);
+    sp = std::max(sp, MIN_MID_BITS);
+    if (sp > MAX_SCALE || (1ull << sp) < bn) return fail(NNH_ERR_ARG, "bad piece scale");
+    const u64 L = 1ull << sp, P = L - bn + 1, n = an + bn;
+    int rc;
+    if (an <= P) {  // one piece: a single product at scale sp
+        Plan pl = make_plan(D, sp);
+        if ((rc = ensure_twiddles(D, sp)) || (rc = ensure(D.ws, 6 * pl.L))) return rc;
+        return product_core(D, pl, a, an, b, bn, false, nullptr, out, n, nullptr, 0, st);
+    }
+    const u64 G = (P + bn + P - 1) / P;
+    if (G > 4) return ntt_chunked_seq(D, out, a, an, b, bn, sp, st);
+    if ((rc = ensure(D.tmp, (G - 1) * n + 64)) || (rc = ensure(D.tiles, 2 * ((n + LIMB_TILE - 1) / LIMB_TILE) + 16)))
+        return rc;
+    auto buf = [&](u64 g) { return g == 0 ? out : D.tmp.p + (g - 1) * n; };
+    for (u64 g = 0; g < G; g++) launch_fill(buf(g), n, 0, st);
+    std::vector<IReq> rq;
+    for (u64 i = 0, off = 0; off < an; i++, off += P) {
+        const u64 pn = std::min(P, an - off);
+        rq.push_back(IReq{buf(i % G) + off, a + off, pn, b, bn, pn + bn, nullptr, 0});
+    }
+    if ((rc = dev_batch(rq.data(), rq.size(), st))) return rc;
+    for (u64 g = 1; g < G; g++)
+        launch_add_one(AddJob{out, out, buf(g), n, n, n, 0, nullptr, nullptr, 0, 0}, (u8*)D.tiles.p, st);
+    HIPCHK(hipGetLastError());
     return NNH_OK;
 }
 

@@ -42,6 +42,25 @@ def test_chunked_piece_scales(nnh, oracle, sp):
     assert np.array_equal(nnh.mul_algo(a, b, nnh.ALGO_NTT_CHUNK, sp), oracle.mul(a, b))
 
 
+# (an, bn, sp): pieces of P = 2^sp - bn + 1 words grouped into G =
+# ceil((P + bn) / P) buffers: G = 2, 3, 4 (schoolbook pieces for bn < 384,
+# transform pieces sharing b's transform above), a last piece shorter than
+# b, and G > 4 (the sequential form)
+CHUNK_SHAPES = [(100000, 300, 10), (100000, 300, 9), (60000, 350, 9), (70001, 1000, 11), (80000, 1200, 11),
+                (80000, 1500, 11), (1 << 20, 5000, 14), (50000, 400, 9), (40000, 1800, 11)]
+
+
+@pytest.mark.parametrize("an,bn,sp", CHUNK_SHAPES)
+def test_chunked_groups(nnh, oracle, an, bn, sp):
+    rng = np.random.default_rng(an + bn + sp)
+    a, b = rand_limbs(rng, an), rand_limbs(rng, bn)
+    P = (1 << sp) - bn + 1
+    say(f"chunked {an}x{bn} sp={sp} G={-(-(P + bn) // P)}")
+    want = oracle.mul(a, b)
+    assert np.array_equal(nnh.mul_algo(a, b, nnh.ALGO_NTT_CHUNK, sp), want)
+    assert np.array_equal(nnh.mul_algo(b, a, nnh.ALGO_NTT_CHUNK, sp), want)   # operands swapped
+
+
 def test_mlp_gradients_vs_finite_differences(nnh):
     from natural_number_amd import costmodel as cm
 
