@@ -23,7 +23,7 @@ N_ARMS = len(ARMS)
 N_IN = 3
 MAX_WORDS = 1 << 24
 TIME_CAP_MAC = 1 << 28     # schoolbook / Karatsuba products beyond this many word-products are not tried
-MAX_PIECES = 256           # chunked transform: at most this many pieces
+MAX_PIECES = 4096          # chunked transform: at most this many pieces (one batch)
 
 
 def _scale(a, b):
