@@ -446,7 +446,7 @@ def main():
     ap.add_argument("--cost-model", action="store_true",
                     help="only run the cost-model pipeline (grid timings, on-device training, held-out regret)")
     ap.add_argument("--held-out", type=int, default=10000, help="held-out requests for --cost-model")
-    ap.add_argument("--grid", type=int, default=20, help="grid points per side for --cost-model")
+    ap.add_argument("--grid", type=int, default=28, help="grid points per side for --cost-model")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
