@@ -14,9 +14,10 @@
  * basic_natural (basic_natural.h:11-13, natural.h:82-93).
  *
  * Errors: the reference never reports one on this path (it exits on
- * allocation failure, basic_natural.h:18); here a non-zero status of the
- * library throws std::runtime_error carrying nnh_last_error().  There is no
- * CPU fallback: without a usable GPU every device operation throws.
+ * allocation failure, basic_natural.h:18; division by zero and a - b with
+ * a < b have defined results, kept here); a non-zero status of the library
+ * throws std::runtime_error carrying nnh_last_error().  There is no CPU
+ * fallback: without a usable GPU every device operation throws.
  */
 #ifndef NNH_NATURAL_HPP
 #define NNH_NATURAL_HPP
@@ -75,10 +76,10 @@ class natural {
         r.resize(n);
         return natural(std::move(r));
     }
-    // q = a / b, r = a % b (the reference's Newton division)
+    // q = a / b, r = a % b (the reference's Newton division).  As in
+    // natural::_div (natural.h:750-756), b = 0 gives q = (a == 0), r = 0.
     static void divmod(const natural& a, const natural& b, natural& q, natural& r) {
         ready();
-        if (b.is_zero()) throw std::domain_error("natural: division by zero");
         const uint64_t qc = a.size() + 1, rc = b.size() + 1;
         std::vector<uint64_t> qv(qc), rv(rc);
         uint64_t qn = 0, rn = 0;
@@ -160,16 +161,19 @@ class natural {
         return natural(std::move(r));
     }
     natural& operator+=(const natural& b) { return *this = *this + b; }
-    // a - b for a >= b (the reference's natural has no negative values)
+    // |a - b|: the reference's natural has no negative values and its
+    // subtraction takes the larger operand first (natural::_sub,
+    // natural.h:324-340).
     friend natural operator-(const natural& a, const natural& b) {
-        if (cmp(a, b) < 0) throw std::domain_error("natural: subtraction below zero");
-        std::vector<uint64_t> r(a.size());
+        const natural& x = cmp(a, b) >= 0 ? a : b;
+        const natural& y = cmp(a, b) >= 0 ? b : a;
+        std::vector<uint64_t> r(x.size());
         unsigned borrow = 0;
-        for (uint64_t i = 0; i < a.size(); i++) {
-            const uint64_t yi = i < b.size() ? b.d_[i] : 0;
-            const uint64_t d = a.d_[i] - yi;
+        for (uint64_t i = 0; i < x.size(); i++) {
+            const uint64_t yi = i < y.size() ? y.d_[i] : 0;
+            const uint64_t d = x.d_[i] - yi;
             r[i] = d - borrow;
-            borrow = (a.d_[i] < yi) | (d < borrow);
+            borrow = (x.d_[i] < yi) | (d < borrow);
         }
         return natural(std::move(r));
     }

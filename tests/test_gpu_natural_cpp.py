@@ -108,7 +108,9 @@ def test_factorial_matches_golden(run):
 
 
 @pytest.mark.gpu
-def test_error_paths_throw(run):
+def test_reference_edge_cases(run):
+    """1 - 2 = 1 (natural.h:324-340 subtracts the smaller from the larger);
+    5 / 0 = 0, 0 / 0 = 1, 5 % 0 = 0 (natural.h:750-756)."""
     out, _ = run
-    assert out["sub_below_zero"] == "throws"
-    assert out["div_by_zero"] == "throws"
+    assert out["sub_smaller"] == "1"
+    assert out["div_by_zero"] == "0 1 0"
