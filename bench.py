@@ -8,9 +8,18 @@ Also reported:
     per-kernel HIP-event time) and of the whole product (B_alg per product);
   * the CPU baseline: the oracle (the restatement of the reference CPU path,
     one thread) on the same product, whose result is also compared
-    word-for-word with the GPU result.
+    word-for-word with the GPU result;
+  * the other plain products at scale (2^30 x 2^22 bits through the
+    reference dispatch and the chunked arm; two all-ones 2^28-bit operands
+    against the closed form);
+  * the batched uneven mix (4096 products, one hipGraph) against one launch
+    sequence per product and against the oracle on the host's cores;
+  * e to 10^6..10^8 digits, pi to 10^7 digits and 10^7! (binary splitting
+    on the device) against their committed digests.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       [--no-batch] [--no-e] [--no-drivers]
+       python bench.py --cost-model [--held-out N] [--grid G]
 """
 import argparse
 import ctypes
