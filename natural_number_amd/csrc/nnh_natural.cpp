@@ -703,10 +703,22 @@ u64 to_dec_into(Ctx& c, DN x, char* out, u64 cap) {
     tick("bases");
     B.rec.resize(I + 1);
     B.n_shl.assign(I + 1, 0);
-    for (int i = DEC_LEAF_LEVEL + 1; i <= I; i++) {  // calc_dec_base (natural.h:909-923)
-        u32 ns = (u32)((std::countl_zero(c.top(B.base[i])) + 1) & 63);
-        B.n_shl[i] = ns;
-        B.rec[i] = reciprocal_plain(c, c.shl(B.base[i], ns), B.base[i].n * 2 + 1);
+    // calc_dec_base (natural.h:909-923) runs the reciprocal's Newton ladder
+    // (natural.h:575-617) for every base; for these divisors it returns
+    // rec[i] = floor(2^(64 S_i) / D_i), D_i = base[i] << n_shl[i],
+    // S_i = 2 n_i + 1.  As base[i+1] = base[i]^2, (base[i] * rec[i+1]) >> t_i
+    // with t_i = 64 (S_{i+1} - S_i) - n_shl[i+1] + n_shl[i] equals
+    // floor(2^(64 S_i) / D_i - e), where e < 2 base[i] / 2^t_i is far below 1
+    // when rec[i+1] is its floor or one less: rec[i] or rec[i] - 1, an
+    // underestimate the split's correction loop absorbs like the ladder's.
+    // Only the top base runs the ladder; each lower base costs one product.
+    auto S = [&](int i) { return 2 * B.base[i].n + 1; };
+    for (int i = DEC_LEAF_LEVEL + 1; i <= I; i++)
+        B.n_shl[i] = (u32)((std::countl_zero(c.top(B.base[i])) + 1) & 63);
+    if (I > DEC_LEAF_LEVEL) B.rec[I] = reciprocal_plain(c, c.shl(B.base[I], B.n_shl[I]), S(I));
+    for (int i = I - 1; i > DEC_LEAF_LEVEL; i--) {
+        const u64 t = 64 * (S(i + 1) - S(i)) - B.n_shl[i + 1] + B.n_shl[i];
+        B.rec[i] = c.shr(c.mul(B.base[i], B.rec[i + 1]), t);
     }
     tick("reciprocals");
     const u64 total = 18ull << (I + 1);

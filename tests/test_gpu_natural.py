@@ -127,7 +127,17 @@ def test_to_dec_powers_of_ten(nnh):
             assert nnh.to_dec(from_int(x, (x.bit_length() + 63) // 64)) == str(x)
 
 
-# ---- e (natural.h:841-857, test.cpp:12-19) ----------------------------------------
+def test_to_dec_largest_pieces_per_level(nnh):
+    """Pieces at the top of each level's range: B_(i+1) - 1 = B_i^2 - 1 and
+    q B_i +- 1, where B_i = 10^(18 2^i) and every base below the top one
+    uses a reciprocal derived from the next base's."""
+    for i in range(5, 12):
+        b = 10 ** (18 << i)
+        for x in (b * b - 1, 999999937 * b - 1, 999999937 * b + 1, (b - 1) * b):
+            assert nnh.to_dec(from_int(x, (x.bit_length() + 63) // 64)) == str(x), i
+
+
+# ---- e (natural.h:841-857, test.cpp:12-19)----------------------------------------
 @pytest.mark.parametrize("k", [6, 10, 14, 17])
 def test_calc_e_matches_oracle(nnh, oracle, k):
     num, den = nnh.calc_e(k)
