@@ -266,8 +266,9 @@ def batch_mix(nn, stream, valu, reps=5, with_cpu=True):
 def e_workload(nn, with_cpu):
     """Config 4: e by binary splitting (reference test.cpp:12-19): calc_e over
     [0, 2^k), times 10^D, the reference's Newton division, decimal output.
-    GPU wall time per digit count, digits checked against the committed
-    digests; the oracle (one thread) is timed on the 10^7-digit case."""
+    GPU wall time per digit count (the second of two calls; the first is
+    reported beside it), digits checked against the committed digests; the
+    oracle (one thread) is timed on the 10^7-digit case."""
     with open(os.path.join(ROOT, "tests", "golden", "e_digits.json")) as f:
         golden = json.load(f)
     log("e: warm-up (10^6 digits)")
@@ -278,14 +279,20 @@ def e_workload(nn, with_cpu):
         log(f"e: {g['digits']} digits on the GPU")
         # timed like the oracle below: the library call writing into a
         # caller's buffer, without the Python-side buffer set-up and decode
+        # two calls: the first grows the library's device buffer pool to this
+        # size, the second (reported) reuses it as a long-running caller would
         buf = ctypes.create_string_buffer(g["digits"] + 64)
-        t = time.perf_counter()
-        n, stages = nn.e_digits_into(buf, g["k"], g["digits"])
-        wall = time.perf_counter() - t
+        walls, ok = [], True
+        for _ in range(2):
+            t = time.perf_counter()
+            n, stages = nn.e_digits_into(buf, g["k"], g["digits"])
+            walls.append(time.perf_counter() - t)
+            ok = ok and hashlib.sha256(buf.raw[:n]).hexdigest() == g["sha256"]
         res[name] = {
-            "k": g["k"], "digits": g["digits"], "gpu_wall_s": round(wall, 4),
+            "k": g["k"], "digits": g["digits"], "gpu_wall_s": round(walls[1], 4),
+            "gpu_wall_s_first_call": round(walls[0], 4),
             "stages_ms": {k: round(v, 3) for k, v in stages.items()},
-            "digits_match_reference": hashlib.sha256(buf.raw[:n]).hexdigest() == g["sha256"],
+            "digits_match_reference": ok,
         }
         del buf
     if with_cpu:
@@ -300,7 +307,8 @@ def e_workload(nn, with_cpu):
         tcpu = time.perf_counter() - t
         ok = hashlib.sha256(buf.raw[:n]).hexdigest() == g["sha256"]
         res["cpu_oracle_e_k21_D10000000"] = {"seconds": round(tcpu, 3), "cores": 1, "digits_match_reference": ok}
-        res["speedup_vs_cpu_at_1e7_digits"] = round(tcpu / res["e_k21_D10000000"]["gpu_wall_s"], 1)
+        # against the first call (buffers allocated inside the timed call, as the oracle's are)
+        res["speedup_vs_cpu_at_1e7_digits"] = round(tcpu / res["e_k21_D10000000"]["gpu_wall_s_first_call"], 1)
     return res
 
 
@@ -319,11 +327,14 @@ def drivers_workload(nn, with_cpu):
         g = gp[name]
         log(f"pi: {g['digits']} digits on the GPU")
         buf = ctypes.create_string_buffer(g["digits"] + 64)  # timed like the oracle: the library call only
-        t = time.perf_counter()
-        n, stages = nn.pi_digits_into(buf, g["k"], g["digits"], g["acc"])
-        wall = time.perf_counter() - t
-        res[name] = {"gpu_wall_s": round(wall, 4), "stages_ms": {k: round(v, 3) for k, v in stages.items()},
-                     "digits_match_reference": hashlib.sha256(buf.raw[:n]).hexdigest() == g["sha256"]}
+        walls, ok = [], True
+        for _ in range(2):  # first call grows the buffer pool; the second is reported
+            t = time.perf_counter()
+            n, stages = nn.pi_digits_into(buf, g["k"], g["digits"], g["acc"])
+            walls.append(time.perf_counter() - t)
+            ok = ok and hashlib.sha256(buf.raw[:n]).hexdigest() == g["sha256"]
+        res[name] = {"gpu_wall_s": round(walls[1], 4), "gpu_wall_s_first_call": round(walls[0], 4),
+                     "stages_ms": {k: round(v, 3) for k, v in stages.items()}, "digits_match_reference": ok}
         del buf
     if with_cpu:
         g = gp["pi_k17_D1000000_acc60000"]
@@ -339,7 +350,8 @@ def drivers_workload(nn, with_cpu):
         res["cpu_oracle_pi_k17_D1000000"] = {"seconds": round(tcpu, 3), "cores": 1,
                                              "digits_match_reference": hashlib.sha256(buf.raw[:n]).hexdigest()
                                              == g["sha256"]}
-        res["pi_speedup_vs_cpu_at_1e6_digits"] = round(tcpu / res["pi_k17_D1000000_acc60000"]["gpu_wall_s"], 1)
+        res["pi_speedup_vs_cpu_at_1e6_digits"] = round(
+            tcpu / res["pi_k17_D1000000_acc60000"]["gpu_wall_s_first_call"], 1)
     n = 10**7
     log("factorial(10^7) on the GPU")
     t = time.perf_counter()

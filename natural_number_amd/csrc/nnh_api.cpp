@@ -1040,6 +1040,7 @@ int nnh_shutdown(void) {
         g_dev.twf[i] = g_dev.twi[i] = nullptr;
     }
     g_dev.tw_scale = 0;
+    host_release();
     for (Buf* b : {&g_dev.ws, &g_dev.tiles, &g_dev.stage, &g_dev.tmp, &g_dev.bws, &g_dev.bjobs, &g_dev.btiles}) {
         if (b->p) (void)hipFree(b->p);
         *b = Buf{};

@@ -14,6 +14,7 @@ namespace nnh {
 int host_init();                  // initialise device 0 if nnh_init was not called
 hipStream_t host_stream();
 std::mutex& host_mutex();
+void host_release();              // free the natural layer's cached device buffers
 int set_error(int code, const std::string& msg);
 
 // out[0 .. an+bn) = a * b, the reference dispatch (natural.h:444-450).

@@ -371,10 +371,14 @@ __global__ __launch_bounds__(256) void k_e_leaves(u64* num, u64* den, u64 nblock
 
 constexpr int E_WG_MAX = 256;
 
+// WG: largest operand (words) the instance holds in LDS; the 512-word
+// instance (46 KB of LDS) takes the levels the 256-word one (23 KB, twice
+// the work-groups per CU) cannot.
+template <int WG>
 __global__ __launch_bounds__(256) void k_e_combine(const u64* __restrict__ num, const u64* __restrict__ den, u64 S,
                                                    u64* num2, u64* den2, u64 S2, u64 ncomb) {
-    __shared__ u64 nl[E_WG_MAX], dl[E_WG_MAX], nr[E_WG_MAX], dr[E_WG_MAX];
-    __shared__ u64 c0[2 * E_WG_MAX], c1[2 * E_WG_MAX], c2[2 * E_WG_MAX];
+    __shared__ u64 nl[WG], dl[WG], nr[WG], dr[WG];
+    __shared__ u64 c0[2 * WG], c1[2 * WG], c2[2 * WG];
     __shared__ u64 xs[3][256 + 3];
     __shared__ u32 wco[2][4];
     __shared__ u8 fn;
@@ -400,8 +404,11 @@ void launch_e_leaves(u64* num, u64* den, u64 nblocks, u64 stride, hipStream_t st
 }
 void launch_e_combine(const u64* num, const u64* den, u64 S, u64* num2, u64* den2, u64 S2, u64 ncomb,
                       hipStream_t st) {
-    hipLaunchKernelGGL(k_e_combine, dim3(persist_grid(ncomb, 16384)), dim3(256), 0, st, num, den, S, num2, den2, S2,
-                       ncomb);
+    const dim3 grid(persist_grid(ncomb, 16384));
+    if (S <= E_WG_MAX)
+        hipLaunchKernelGGL(k_e_combine<E_WG_MAX>, grid, dim3(256), 0, st, num, den, S, num2, den2, S2, ncomb);
+    else
+        hipLaunchKernelGGL(k_e_combine<2 * E_WG_MAX>, grid, dim3(256), 0, st, num, den, S, num2, den2, S2, ncomb);
 }
 
 // ---------------------------------------------------------------------------

@@ -52,7 +52,7 @@ void launch_div_word(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64* rem,
 // Leaves: thread j folds the 16 pairs [32j, 32j+32) into (num, den) at
 // stride words.  Requires stride >= the value's words (<= 14 for terms < 2^25).
 void launch_e_leaves(u64* num, u64* den, u64 nblocks, u64 stride, hipStream_t st);
-// One level: combine c folds ranges 2c and 2c+1 (stride S <= 256) into
+// One level: combine c folds ranges 2c and 2c+1 (stride S <= 512) into
 // num2 = num_l den_r + num_r, den2 = den_l den_r (stride S2 <= 4096).
 void launch_e_combine(const u64* num, const u64* den, u64 S, u64* num2, u64* den2, u64 S2, u64 ncomb,
                       hipStream_t st);

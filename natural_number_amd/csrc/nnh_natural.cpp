@@ -39,6 +39,7 @@ constexpr u64 DEC_BASE0 = 1000000000000000000ull;  // natural.h:904
 constexpr int DEC_WG_MAX_WORDS = 256;    // bases split by one work-group (kernel limit)
 constexpr int DEC_LEAF_LEVEL = 3;        // pieces < 10^(18*16) become digits per thread
 constexpr int E_WG_MAX_WORDS = 256;      // e levels combined by one work-group
+constexpr int E_COMBINE_MAX_WORDS = 512;  // e levels whose combines run one per work-group
 constexpr int E_LEAF_MAX_WORDS = 14;
 
 int ntt_scale(u64 s1, u64 s2) { return 64 - std::countl_zero(s1 + s2 - 2); }  // natural.h:384-386
@@ -61,13 +62,18 @@ void rcchk(int rc) {
     if (rc) throw Err{rc, nnh_last_error()};
 }
 
-// Device buffers recycled by power-of-two capacity; all released when the
-// pool dies.  Reuse is safe because every kernel runs on one stream.
+// Device buffers recycled by power-of-two capacity.  One pool serves every
+// call for the life of the process (calls hold host_mutex() and end with
+// their stream synchronised, so a buffer is reused only after the work that
+// used it); nnh_shutdown releases it.  Within a call, reuse is safe because
+// every kernel runs on one stream.
 class Pool {
    public:
-    ~Pool() {
+    void release() {
         (void)hipDeviceSynchronize();
         for (u64* p : all_) (void)hipFree(p);
+        all_.clear();
+        free_.clear();
     }
     u64* get(u64 words, u64& cap) {
         cap = std::bit_ceil(std::max<u64>(words, 64));
@@ -88,6 +94,12 @@ class Pool {
     std::multimap<u64, u64*> free_;
     std::vector<u64*> all_;
 };
+
+// Never destroyed: at process exit the HIP runtime may already be gone.
+Pool& shared_pool() {
+    static Pool* p = new Pool;
+    return *p;
+}
 
 struct Block {
     Pool* pool;
@@ -122,7 +134,7 @@ struct Xf {
 
 class Ctx {
    public:
-    explicit Ctx(hipStream_t s) : st(s) {
+    explicit Ctx(hipStream_t s) : st(s), pool(shared_pool()) {
         hipchk(hipHostMalloc((void**)&h_, 4096, hipHostMallocDefault), "pinned allocation");
         res_ = zeros(8);
         mailbox();
@@ -135,7 +147,7 @@ class Ctx {
         if (h_jobs_) (void)hipHostFree(h_jobs_);
     }
     hipStream_t st;
-    Pool pool;
+    Pool& pool;
 
     DN alloc(u64 n) {
         u64 cap;
@@ -815,7 +827,7 @@ void calc_e(Ctx& c, int k, DN& num, DN& den) {
         }
         u64 S2 = e_stride(k, l), ncomb = 1ull << (k - l);
         DN nn = c.alloc(ncomb * S2), nd = c.alloc(ncomb * S2);
-        if (S <= E_WG_MAX_WORDS && S2 <= 4096) {
+        if (S <= E_COMBINE_MAX_WORDS && S2 <= 4096) {
             launch_e_combine(cn.p, cd.p, S, nn.p, nd.p, S2, ncomb, c.st);
         } else {
             std::vector<IReq> rq(2 * ncomb);
@@ -1015,6 +1027,9 @@ u64 emit(Ctx& c, const DN& x, u64* out, u64 cap) {
 }
 
 }  // namespace
+
+void host_release() { shared_pool().release(); }
+
 }  // namespace nnh
 
 using namespace nnh;
