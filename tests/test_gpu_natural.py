@@ -226,3 +226,16 @@ def test_pi_digits_golden(nnh, name):
     s, _ = nnh.pi_digits(g["k"], g["digits"], g["acc"])
     assert len(s) == g["length"] and s[:60] == g["head"] and s[-60:] == g["tail"]
     assert hashlib.sha256(s.encode()).hexdigest() == g["sha256"]
+
+
+def test_shutdown_releases_and_recovers(nnh, oracle):
+    """nnh_shutdown frees the twiddle tables, workspaces and the natural
+    layer's buffer pool; the next calls rebuild them and give the same
+    results."""
+    a = rand_limbs(np.random.default_rng(77), 5000)
+    before = (nnh.e_digits(12, 3000)[0], nnh.to_dec(a), to_int(nnh.mul(a, a)))
+    assert nnh.lib().nnh_shutdown() == 0
+    after = (nnh.e_digits(12, 3000)[0], nnh.to_dec(a), to_int(nnh.mul(a, a)))
+    assert after == before
+    assert after[0] == oracle.e_digits(12, 3000) and after[1] == oracle.to_dec(a)
+    assert after[2] == to_int(a) ** 2
