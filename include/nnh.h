@@ -40,8 +40,16 @@ typedef enum nnh_status {
  * Reference: the static tables of NTT_info (ntt_info.h:69-145). */
 int nnh_init(int device);
 
-/* Release every device buffer held by the library. */
+/* Release every device buffer held by the library.  Batches that are still
+ * alive keep the twiddle tables they read until nnh_batch_destroy. */
 int nnh_shutdown(void);
+
+/* Test hook (no counterpart in the reference): the next n compute calls
+ * (products, batches, transforms, natural-layer calls) return NNH_ERR_HIP
+ * before anything is queued, so that a caller's fallback to its CPU path can
+ * be exercised; n = 0 cancels.  Counts across threads.  Returns how many
+ * injected failures were still pending before this call. */
+int nnh_debug_fail_next(int n);
 
 /* dst[0 .. an+bn) = a * b for host buffers.  dst is not normalised: the
  * caller trims top zero words (natural::std, natural.h:82-93).  dst may

@@ -46,6 +46,7 @@ def lib():
     sigs = {
         "nnh_init": ([i32], i32),
         "nnh_shutdown": ([], i32),
+        "nnh_debug_fail_next": ([i32], i32),
         "nnh_mul": ([_u64p, _u64p, u64, _u64p, u64], i32),
         "nnh_sqr": ([_u64p, _u64p, u64], i32),
         "nnh_mul_device_async": ([vp, vp, u64, vp, u64, vp], i32),

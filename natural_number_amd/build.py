@@ -88,5 +88,59 @@ def build_natural_check(verbose=False):
     return exe
 
 
+def extract_bindings(md_path, out_path):
+    """Every cpp code block of INTEGRATION.md's binding section, concatenated
+    into one include file (the bindings a maintainer would add to the
+    reference's natural.h)."""
+    text = open(md_path).read()
+    head = "## The binding a maintainer would add to natural.h"
+    start = text.index(head) + len(head)
+    nxt = text.find("\n## ", start)
+    sec = text[start:nxt if nxt >= 0 else len(text)]
+    blocks = []
+    pos = 0
+    while True:
+        i = sec.find("```cpp\n", pos)
+        if i < 0:
+            break
+        j = sec.index("\n```", i + 7)
+        blocks.append(sec[i + 7:j + 1])
+        pos = j + 4
+    if not blocks:
+        raise RuntimeError("no cpp blocks in the binding section of " + md_path)
+    body = "// Generated from INTEGRATION.md by natural_number_amd/build.py; do not edit.\n" + "\n".join(blocks)
+    tmp = out_path + ".tmp"
+    with open(tmp, "w") as f:
+        f.write(body)
+    os.replace(tmp, out_path)
+    return len(blocks)
+
+
+def build_binding_check(verbose=False, out_dir=None):
+    """The INTEGRATION.md bindings compiled against a stand-in natural whose
+    CPU bodies are the oracle (tests/cpp/binding_check.cpp; test
+    infrastructure, run by tests/test_bindings.py)."""
+    root = os.path.dirname(HERE)
+    src_dir = os.path.join(root, "tests", "cpp")
+    out_dir = out_dir or src_dir
+    inc = os.path.join(out_dir, "binding_snippets.inc")
+    extract_bindings(os.path.join(root, "INTEGRATION.md"), inc)
+    exe = os.path.join(out_dir, "binding_check")
+    cmd = ["g++", "-O2", "-std=c++20", "-march=x86-64-v3", "-Wall", "-Wextra", "-Wno-unused-function",
+           os.path.join(src_dir, "binding_check.cpp"), os.path.join(root, "oracle", "nn_oracle.cpp"),
+           "-o", exe + ".tmp", "-I", out_dir, "-I", os.path.join(root, "include"), "-I", os.path.join(root, "oracle"),
+           "-L", HERE, "-lnnh", "-Wl,-rpath," + HERE]
+    if out_dir == src_dir:
+        cmd[-1] = "-Wl,-rpath,$ORIGIN/../../natural_number_amd"
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=src_dir, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -11,10 +11,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <bit>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -66,6 +68,24 @@ int set_error(int code, const std::string& msg) { return fail(code, msg); }
 struct Buf {
     u64* p = nullptr;
     u64 words = 0;
+    bool fixed = false;  // caller-provided memory of a set size: never freed or grown
+};
+
+// Montgomery-form twiddle tables of the three primes for transforms up to
+// 2^scale words.  Shared: a batch keeps the tables its launches (or its
+// captured graph) read, so growing the device's tables for a larger
+// product, or nnh_shutdown, leaves them alive until the batch is destroyed.
+struct Twiddles {
+    int scale = 0;
+    u64* f[3]{};
+    u64* i[3]{};
+    ~Twiddles() {
+        (void)hipDeviceSynchronize();  // queued work on any stream may still read them
+        for (int k = 0; k < 3; k++) {
+            if (f[k]) (void)hipFree(f[k]);
+            if (i[k]) (void)hipFree(i[k]);
+        }
+    }
 };
 
 struct Device {
@@ -94,8 +114,16 @@ struct Device {
     CrtConsts crt{};
     u64 rb_plain[3][56]{}, irb_plain[3][56]{};
     int tw_scale = 0;          // tables hold 2^(tw_scale-2) entries
-    u64* twf[3]{};
-    u64* twi[3]{};
+    const u64* twf[3]{};       // the tables the next launches read (tw's, or fixed ones)
+    const u64* twi[3]{};
+    std::shared_ptr<Twiddles> tw;
+    bool tw_fixed = false;     // twf/twi are fixed tables of tw_scale: never grown
+    // The workspace below is shared by every call on the device.  Each call
+    // records ev_ws on its stream after its last launch; a call on another
+    // stream waits for that event before its first launch, so queued work of
+    // two calls never overlaps in the workspace.
+    hipEvent_t ev_ws = nullptr;
+    hipStream_t ws_stream = nullptr;  // stream ev_ws was last recorded on (null: none pending)
     Buf ws;                    // 6 L words: A and B transforms
     Buf tiles;                 // tile carry functions and carry-ins (bytes)
     Buf stage;                 // host-API staging: a, b, dst
@@ -159,13 +187,27 @@ static void prof_collect() {
 
 static int ensure(Buf& b, u64 words) {
     if (b.words >= words) return NNH_OK;
+    if (b.fixed)
+        return fail(NNH_ERR_ARG, "workspace of " + std::to_string(words) + " words exceeds the fixed limit of " +
+                                     std::to_string(b.words));
     // queued work on any stream may still use the old buffer
     if (b.p) HIPCHK(hipDeviceSynchronize());
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.words = 0;
     u64 w = std::bit_ceil(std::max<u64>(words, 1024));
-    HIPCHK(hipMalloc((void**)&b.p, w * sizeof(u64)));
+    hipError_t e = hipMalloc((void**)&b.p, w * sizeof(u64));
+    if (e == hipErrorOutOfMemory) {
+        // idle buffers cached by the natural layer's pool may hold the memory
+        (void)hipGetLastError();
+        host_trim();
+        e = hipMalloc((void**)&b.p, w * sizeof(u64));
+    }
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? NNH_ERR_NOMEM : NNH_ERR_HIP,
+                    std::string("workspace allocation: ") + hipGetErrorString(e));
+    }
     b.words = w;
     return NNH_OK;
 }
@@ -208,24 +250,68 @@ static void build_constants(Device& D) {
 
 static int ensure_twiddles(Device& D, int s) {
     if (s <= D.tw_scale) return NNH_OK;
+    if (D.tw_fixed)
+        return fail(NNH_ERR_ARG, "transform of 2^" + std::to_string(s) + " words exceeds the fixed twiddle tables (2^" +
+                                     std::to_string(D.tw_scale) + ")");
     int ns = std::max(s, 12);
     u64 n = 1ull << (ns - 2);
-    if (D.tw_scale) HIPCHK(hipDeviceSynchronize());
+    auto T = std::make_shared<Twiddles>();
     for (int i = 0; i < 3; i++) {
-        if (D.twf[i]) (void)hipFree(D.twf[i]);
-        if (D.twi[i]) (void)hipFree(D.twi[i]);
-        D.twf[i] = D.twi[i] = nullptr;
-    }
-    D.tw_scale = 0;
-    for (int i = 0; i < 3; i++) {
-        HIPCHK(hipMalloc((void**)&D.twf[i], n * sizeof(u64)));
-        HIPCHK(hipMalloc((void**)&D.twi[i], n * sizeof(u64)));
-        launch_twiddles(D.twf[i], n, D.rb, i, 0, D.stream);
-        launch_twiddles(D.twi[i], n, D.rb, i, 1, D.stream);
+        HIPCHK(hipMalloc((void**)&T->f[i], n * sizeof(u64)));
+        HIPCHK(hipMalloc((void**)&T->i[i], n * sizeof(u64)));
+        launch_twiddles(T->f[i], n, D.rb, i, 0, D.stream);
+        launch_twiddles(T->i[i], n, D.rb, i, 1, D.stream);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(D.stream));
+    T->scale = ns;
+    // the previous tables are freed here unless a batch still holds them
+    D.tw = std::move(T);
+    for (int i = 0; i < 3; i++) {
+        D.twf[i] = D.tw->f[i];
+        D.twi[i] = D.tw->i[i];
+    }
     D.tw_scale = ns;
+    return NNH_OK;
+}
+
+static void drop_twiddles(Device& D) {
+    D.tw.reset();
+    for (int i = 0; i < 3; i++) D.twf[i] = D.twi[i] = nullptr;
+    D.tw_scale = 0;
+}
+
+// Workspace hand-over between calls (see Device::ev_ws).  Streams being
+// captured into a graph are left alone: their work runs when the caller
+// replays the graph, which the caller orders.
+static bool capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+static int ws_acquire(Device& D, hipStream_t st) {
+    if (!D.ws_stream || D.ws_stream == st || capturing(st)) return NNH_OK;
+    HIPCHK(hipStreamWaitEvent(st, D.ev_ws, 0));
+    return NNH_OK;
+}
+static void ws_release(Device& D, hipStream_t st) {
+    if (capturing(st)) return;
+    if (hipEventRecord(D.ev_ws, st) == hipSuccess) D.ws_stream = st;
+}
+struct WsScope {
+    Device& D;
+    hipStream_t st;
+    int rc;
+    WsScope(Device& d, hipStream_t s) : D(d), st(s), rc(ws_acquire(d, s)) {}
+    ~WsScope() { ws_release(D, st); }
+};
+
+// Test hook: the next n compute calls fail with NNH_ERR_HIP before they
+// enqueue anything (nnh_debug_fail_next).
+static std::atomic<int> g_fail_next{0};
+int injected_failure() {
+    int n = g_fail_next.load();
+    while (n > 0)
+        if (g_fail_next.compare_exchange_weak(n, n - 1)) return fail(NNH_ERR_HIP, "injected failure (nnh_debug_fail_next)");
     return NNH_OK;
 }
 
@@ -240,6 +326,7 @@ static int device_init(int dev) {
     if (!g_dev.stream2) HIPCHK(hipStreamCreateWithFlags(&g_dev.stream2, hipStreamNonBlocking));
     if (!g_dev.ev_fork) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_fork, hipEventDisableTiming));
     if (!g_dev.ev_join) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_join, hipEventDisableTiming));
+    if (!g_dev.ev_ws) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_ws, hipEventDisableTiming));
     {
         const char* e = std::getenv("NNH_BATCH_STREAMS");
         if (e) g_dev.nwork = std::max(0, std::min(Device::NWORK, std::atoi(e)));
@@ -718,6 +805,11 @@ struct nnh_batch {
     nnh::u64* ws = nullptr;
     nnh::u64 ws_words = 0;
     int max_s = 0;
+    // twiddle tables read by the batch's launches (and baked into its graph),
+    // kept alive by tw for as long as the batch exists
+    std::shared_ptr<nnh::Twiddles> tw;
+    const nnh::u64* twf[3]{};
+    const nnh::u64* twi[3]{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t cap_stream = nullptr;
@@ -766,15 +858,15 @@ static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
         std::vector<std::pair<int, int>> passes = top_passes(pl);
         for (int op = 0; op < 2; op++)
             for (size_t k = 0; k < passes.size(); k++) {
-                for (int i = 0; i < 3; i++) tp.tw[i] = D.twf[i];
+                for (int i = 0; i < 3; i++) tp.tw[i] = B->twf[i];
                 tp.g_lo = (u32)passes[k].first;
                 tp.nlev = (u32)passes[k].second;
                 launch_top_pass_batch(jobs, bk.count, op, tp, false, k == 0, sa, sb);
             }
         MidBatchParams mp{};
         for (int i = 0; i < 3; i++) {
-            mp.twf[i] = D.twf[i];
-            mp.twi[i] = D.twi[i];
+            mp.twf[i] = B->twf[i];
+            mp.twi[i] = B->twi[i];
             mp.alpha_m[i] = pl.alpha_m[i];
         }
         mp.L = pl.L;
@@ -782,7 +874,7 @@ static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
             launch_mid_batch(B->d_jobs + bk.own_first, bk.own_count, mp, pl.cb, pl.ntop == 0, true, sa, sb);
         launch_mid_batch(jobs, bk.count, mp, pl.cb, pl.ntop == 0, false, sa, sb);
         for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
-            for (int i = 0; i < 3; i++) tp.tw[i] = D.twi[i];
+            for (int i = 0; i < 3; i++) tp.tw[i] = B->twi[i];
             tp.g_lo = (u32)it->first;
             tp.nlev = (u32)it->second;
             launch_top_pass_batch(jobs, bk.count, 0, tp, true, false, sa, sb);
@@ -909,6 +1001,11 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
         batch_free(B);
         return rc;
     }
+    B->tw = D.tw;
+    for (int i = 0; i < 3; i++) {
+        B->twf[i] = D.twf[i];
+        B->twi[i] = D.twi[i];
+    }
     if (flags & 2) {
         // internal batch: carve the buffers out of the device's reusable workspace
         B->borrowed = true;
@@ -953,9 +1050,13 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
         HIPCHK(hipStreamCreateWithFlags(&B->cap_stream, hipStreamNonBlocking));
         HIPCHK(hipStreamBeginCapture(B->cap_stream, hipStreamCaptureModeThreadLocal));
         const int erc = batch_enqueue(D, B, B->cap_stream);
-        HIPCHK(hipStreamEndCapture(B->cap_stream, &B->graph));
-        if (erc) return erc;
-        HIPCHK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        const hipError_t ce = hipStreamEndCapture(B->cap_stream, &B->graph);
+        const hipError_t ie = (erc || ce != hipSuccess) ? hipSuccess : hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0);
+        if (erc || ce != hipSuccess || ie != hipSuccess) {
+            batch_free(B);
+            if (erc) return erc;
+            return fail(NNH_ERR_HIP, std::string("batch graph: ") + hipGetErrorString(ce != hipSuccess ? ce : ie));
+        }
     }
     *out = B;
     return NNH_OK;
@@ -967,6 +1068,8 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
 int host_init() { return g_inited ? NNH_OK : device_init(0); }
 hipStream_t host_stream() { return g_dev.stream; }
 std::mutex& host_mutex() { return g_dev.mu; }
+int host_ws_acquire(hipStream_t st) { return ws_acquire(g_dev, st); }
+void host_ws_release(hipStream_t st) { ws_release(g_dev, st); }
 
 int dev_mul(u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st) {
     return mul_device(g_dev, out, a, an, b, bn, st);
@@ -1027,19 +1130,29 @@ using namespace nnh;
         }                                                                        \
     } while (0)
 
+// Start of a compute call on stream ST_: initialise, take the device lock,
+// honour an injected failure, and order this call's queued work after the
+// previous call's (the shared workspace).  Defines `st`.
+#define ENTER(ST_)                                                               \
+    NEED_INIT();                                                                 \
+    std::lock_guard<std::mutex> lock_(g_dev.mu);                                 \
+    if (int rc_ = injected_failure()) return rc_;                                \
+    hipStream_t st = (ST_);                                                      \
+    WsScope ws_scope_(g_dev, st);                                                \
+    if (ws_scope_.rc) return ws_scope_.rc
+
 extern "C" {
 
 int nnh_init(int device) { return device_init(device); }
 
+int nnh_debug_fail_next(int n) { return g_fail_next.exchange(n > 0 ? n : 0); }
+
 int nnh_shutdown(void) {
     std::lock_guard<std::mutex> g(g_dev.mu);
     if (!g_inited) return NNH_OK;
-    for (int i = 0; i < 3; i++) {
-        if (g_dev.twf[i]) (void)hipFree(g_dev.twf[i]);
-        if (g_dev.twi[i]) (void)hipFree(g_dev.twi[i]);
-        g_dev.twf[i] = g_dev.twi[i] = nullptr;
-    }
-    g_dev.tw_scale = 0;
+    (void)hipDeviceSynchronize();
+    g_dev.ws_stream = nullptr;
+    drop_twiddles(g_dev);  // batches still alive keep the tables they use
     host_release();
     for (Buf* b : {&g_dev.ws, &g_dev.tiles, &g_dev.stage, &g_dev.tmp, &g_dev.bws, &g_dev.bjobs, &g_dev.btiles}) {
         if (b->p) (void)hipFree(b->p);
@@ -1078,9 +1191,7 @@ const char* nnh_last_error(void) { return g_err.c_str(); }
 int nnh_mul_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
                          void* stream) {
     if (!d_dst || !d_a || !d_b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
-    hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
+    ENTER(stream ? (hipStream_t)stream : g_dev.stream);
     int rc = mul_device(g_dev, d_dst, d_a, an, d_b, bn, st);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
@@ -1089,9 +1200,7 @@ int nnh_mul_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, cons
 
 int nnh_sqr_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, void* stream) {
     if (!d_dst || !d_a || !an) return fail(NNH_ERR_ARG, "null pointer or empty operand");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
-    hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
+    ENTER(stream ? (hipStream_t)stream : g_dev.stream);
     int rc = sqr_device(g_dev, d_dst, d_a, an, st);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
@@ -1100,15 +1209,13 @@ int nnh_sqr_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, void
 
 int nnh_mul(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn) {
     if (!dst || !a || !b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
+    ENTER(g_dev.stream);
     Device& D = g_dev;
     int rc = stage_host(D, 2 * (an + bn) + 16);
     if (rc) return rc;
     u64* da = D.stage.p;
     u64* db = da + an;
     u64* dd = db + bn;
-    hipStream_t st = D.stream;
     HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
     if ((rc = mul_device(D, dd, da, an, db, bn, st))) return rc;
@@ -1121,9 +1228,7 @@ int nnh_mul(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, ui
 int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
                               int algo, int param, void* stream) {
     if (!d_dst || !d_a || !d_b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
-    hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
+    ENTER(stream ? (hipStream_t)stream : g_dev.stream);
     int rc = mul_algo_device(g_dev, d_dst, d_a, an, d_b, bn, algo, param, st);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
@@ -1158,10 +1263,8 @@ int nnh_time_algo(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint6
                   int param, int count, int reps, float* ms_per_product) {
     if (!d_dst || !d_a || !d_b || !an || !bn || count < 1 || reps < 1 || !ms_per_product)
         return fail(NNH_ERR_ARG, "bad timing arguments");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
+    ENTER(g_dev.stream);
     Device& D = g_dev;
-    hipStream_t st = D.stream;
     int rc = mul_algo_device(D, d_dst, d_a, an, d_b, bn, algo, param, st);  // warm-up, grows buffers
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(st));
@@ -1184,15 +1287,13 @@ int nnh_time_algo(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint6
 
 int nnh_mul_algo(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn, int algo, int param) {
     if (!dst || !a || !b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
+    ENTER(g_dev.stream);
     Device& D = g_dev;
     int rc = stage_host(D, 2 * (an + bn) + 16);
     if (rc) return rc;
     u64* da = D.stage.p;
     u64* db = da + an;
     u64* dd = db + bn;
-    hipStream_t st = D.stream;
     HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
     if ((rc = mul_algo_device(D, dd, da, an, db, bn, algo, param, st))) return rc;
@@ -1204,14 +1305,12 @@ int nnh_mul_algo(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* 
 
 int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an) {
     if (!dst || !a || !an) return fail(NNH_ERR_ARG, "null pointer or empty operand");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
+    ENTER(g_dev.stream);
     Device& D = g_dev;
     int rc = stage_host(D, 3 * an + 16);
     if (rc) return rc;
     u64* da = D.stage.p;
     u64* dd = da + an;
-    hipStream_t st = D.stream;
     HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
     if ((rc = sqr_device(D, dd, da, an, st))) return rc;
     HIPCHK(hipGetLastError());
@@ -1259,7 +1358,16 @@ int nnh_stream_create(void** stream) {
     return NNH_OK;
 }
 int nnh_stream_destroy(void* stream) {
-    if (stream) HIPCHK(hipStreamDestroy((hipStream_t)stream));
+    if (!stream) return NNH_OK;
+    {
+        // a later stream may reuse the address: the workspace hand-over must not mistake it for this one
+        std::lock_guard<std::mutex> g(g_dev.mu);
+        if (g_dev.ws_stream == (hipStream_t)stream) {
+            HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+            g_dev.ws_stream = nullptr;
+        }
+    }
+    HIPCHK(hipStreamDestroy((hipStream_t)stream));
     return NNH_OK;
 }
 int nnh_synchronize(void* stream) {
@@ -1295,6 +1403,7 @@ int nnh_batch_create(const nnh_req* reqs, uint64_t n, int flags, nnh_batch** out
     NEED_INIT();
     std::lock_guard<std::mutex> g(g_dev.mu);
     *out = nullptr;
+    if (int rc = injected_failure()) return rc;
     std::vector<IReq> ir(n);
     for (u64 i = 0; i < n; i++)
         ir[i] = IReq{reqs[i].dst, reqs[i].a, reqs[i].an, reqs[i].b, reqs[i].bn, reqs[i].an + reqs[i].bn, nullptr, 0};
@@ -1303,8 +1412,7 @@ int nnh_batch_create(const nnh_req* reqs, uint64_t n, int flags, nnh_batch** out
 
 int nnh_batch_run(nnh_batch* b, void* stream) {
     if (!b) return fail(NNH_ERR_ARG, "null batch");
-    std::lock_guard<std::mutex> g(g_dev.mu);
-    hipStream_t st = stream ? (hipStream_t)stream : g_dev.stream;
+    ENTER(stream ? (hipStream_t)stream : g_dev.stream);
     if (b->exec) HIPCHK(hipGraphLaunch(b->exec, st));
     else if (int rc = batch_enqueue(g_dev, b, st)) return rc;
     HIPCHK(hipGetLastError());
@@ -1340,14 +1448,12 @@ int nnh_alpha(int scale, uint64_t out[3]) {
 int nnh_transform_forward(uint64_t* out, const uint64_t* a, uint64_t an, int scale) {
     if (!out || !a || !an || scale < MIN_MID_BITS || scale > MAX_SCALE || an > (1ull << scale))
         return fail(NNH_ERR_ARG, "bad transform arguments");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
+    ENTER(g_dev.stream);
     Device& D = g_dev;
     Plan pl = make_plan(D, scale);
     int rc;
     if ((rc = ensure_twiddles(D, scale)) || (rc = ensure(D.ws, 6 * pl.L)) || (rc = stage_host(D, an + 16)))
         return rc;
-    hipStream_t st = D.stream;
     HIPCHK(hipMemcpyAsync(D.stage.p, a, an * 8, hipMemcpyHostToDevice, st));
     if ((rc = transform_full(D, pl, D.stage.p, an, D.ws.p, st))) return rc;
     HIPCHK(hipGetLastError());
@@ -1361,15 +1467,13 @@ int nnh_mul_transformed(uint64_t* dst, const uint64_t* fa, uint64_t an, const ui
     if (!dst || !fa || !b || !an || !bn || scale < MIN_MID_BITS || scale > MAX_SCALE ||
         an + bn - 1 > (1ull << scale))
         return fail(NNH_ERR_ARG, "bad arguments");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
+    ENTER(g_dev.stream);
     Device& D = g_dev;
     Plan pl = make_plan(D, scale);
     int rc;
     if ((rc = ensure_twiddles(D, scale)) || (rc = ensure(D.ws, 6 * pl.L)) ||
         (rc = stage_host(D, bn + an + bn + 16)))
         return rc;
-    hipStream_t st = D.stream;
     u64* fsaved = D.ws.p + 3 * pl.L;  // the reused transform plays operand B
     u64* db = D.stage.p;
     u64* dd = db + bn;
@@ -1387,8 +1491,7 @@ int nnh_mul_cyclic(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t
         return fail(NNH_ERR_ARG, "bad arguments");
     u64 L = 1ull << scale;
     if (an > L || bn > L) return fail(NNH_ERR_ARG, "operand longer than the transform");
-    NEED_INIT();
-    std::lock_guard<std::mutex> g(g_dev.mu);
+    ENTER(g_dev.stream);
     Device& D = g_dev;
     Plan pl = make_plan(D, scale);
     bool cyclic = an + bn - 1 > L;
@@ -1397,7 +1500,6 @@ int nnh_mul_cyclic(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t
     if ((rc = ensure_twiddles(D, scale)) || (rc = ensure(D.ws, 6 * pl.L)) ||
         (rc = stage_host(D, an + bn + n_out + 16)))
         return rc;
-    hipStream_t st = D.stream;
     u64* da = D.stage.p;
     u64* db = da + an;
     u64* dd = db + bn;

@@ -15,7 +15,14 @@ int host_init();                  // initialise device 0 if nnh_init was not cal
 hipStream_t host_stream();
 std::mutex& host_mutex();
 void host_release();              // free the natural layer's cached device buffers
+void host_trim();                 // free the natural layer's idle cached buffers (caller holds host_mutex())
 int set_error(int code, const std::string& msg);
+// Order a call's queued work after the previous call's on another stream,
+// and record the end of this call's work (the shared workspace).
+int host_ws_acquire(hipStream_t st);
+void host_ws_release(hipStream_t st);
+// NNH_ERR_HIP (with its message set) when nnh_debug_fail_next asked for a failure.
+int injected_failure();
 
 // out[0 .. an+bn) = a * b, the reference dispatch (natural.h:444-450).
 int dev_mul(u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st);

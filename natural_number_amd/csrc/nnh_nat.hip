@@ -715,11 +715,13 @@ __global__ __launch_bounds__(256) void k_dec_split_wg(const u64* __restrict__ x,
         wg_mul(Pq, 2 * w, H, w, Bs, w, nullptr, 0, c0, c1, c2, xs, wco, &fn);
         u32 nb = wg_addsub(X, 2 * w, Pq, 2 * w, 1, 1, wco);  // lo = x - hi*base
         int bad = nb ? 0 : 1;
-        for (int it = 0; it < 8 && wg_cmp(X, 2 * w, Bs, w, &sh) >= 0; it++) {
+        // at most 8 corrections, the bound the batched split levels apply
+        int it = 0;
+        for (; it < 8 && wg_cmp(X, 2 * w, Bs, w, &sh) >= 0; it++) {
             wg_addsub(X, 2 * w, Bs, w, 1, 1, wco);
             wg_addsub(H, w, nullptr, 0, 0, 1, wco);
-            if (it == 7) bad = 1;
         }
+        if (it == 8 && wg_cmp(X, 2 * w, Bs, w, &sh) >= 0) bad = 1;
         if (bad && t == 0) *err = 1;
         for (u64 k = t; k < w; k += 256) {
             out[(2 * j) * w + k] = H[k];

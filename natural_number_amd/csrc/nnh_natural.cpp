@@ -75,6 +75,16 @@ class Pool {
         all_.clear();
         free_.clear();
     }
+    // Free the idle buffers (those not handed out).
+    void trim() {
+        if (free_.empty()) return;
+        (void)hipDeviceSynchronize();
+        for (auto& [cap, p] : free_) {
+            (void)hipFree(p);
+            all_.erase(std::find(all_.begin(), all_.end(), p));
+        }
+        free_.clear();
+    }
     u64* get(u64 words, u64& cap) {
         cap = std::bit_ceil(std::max<u64>(words, 64));
         auto it = free_.find(cap);
@@ -84,7 +94,14 @@ class Pool {
             return p;
         }
         u64* p = nullptr;
-        hipchk(hipMalloc((void**)&p, cap * sizeof(u64)), "device allocation");
+        hipError_t e = hipMalloc((void**)&p, cap * sizeof(u64));
+        if (e == hipErrorOutOfMemory && !free_.empty()) {
+            // idle buffers of other size classes may hold the memory
+            (void)hipGetLastError();
+            trim();
+            e = hipMalloc((void**)&p, cap * sizeof(u64));
+        }
+        hipchk(e, "device allocation");
         all_.push_back(p);
         return p;
     }
@@ -1007,13 +1024,17 @@ int guarded(F&& f) {
     int rc = host_init();
     if (rc) return rc;
     std::lock_guard<std::mutex> g(host_mutex());
+    if ((rc = injected_failure())) return rc;
+    if ((rc = host_ws_acquire(host_stream()))) return rc;
     try {
         Ctx c(host_stream());
         f(c);
         c.sync();
+        host_ws_release(host_stream());
         return NNH_OK;
     } catch (const Err& e) {
         (void)hipStreamSynchronize(host_stream());
+        host_ws_release(host_stream());
         return set_error(e.code, e.msg);
     } catch (const std::bad_alloc&) {
         return set_error(NNH_ERR_NOMEM, "host allocation failed");
@@ -1029,6 +1050,7 @@ u64 emit(Ctx& c, const DN& x, u64* out, u64 cap) {
 }  // namespace
 
 void host_release() { shared_pool().release(); }
+void host_trim() { shared_pool().trim(); }
 
 }  // namespace nnh
 

@@ -1,0 +1,320 @@
+// binding_check.cpp — TEST INFRASTRUCTURE.  Compiles the reference-side
+// bindings shown in INTEGRATION.md (every cpp block of its binding section,
+// extracted into binding_snippets.inc by natural_number_amd/build.py) against
+// a stand-in `natural`, and checks them against the oracle.
+//
+// The stand-in keeps the semantics the bindings depend on:
+//   * basic_natural (basic_natural.h:9-97): malloc'd limbs, power-of-two
+//     capacity, resize() sets size and grows by realloc, move assignment
+//     frees the target and takes the source's buffer;
+//   * natural's operators (natural.h:537-564, 771-793): a *= b is
+//     a._mul(&a, &b); a /= b is a._div(&a, &b, &r); a %= b is
+//     q._div(&a, &b, &a) (the remainder aliases the dividend);
+//   * the CPU bodies (_mul_base, _mul_NTT, _sqr_base, _sqr_NTT, the
+//     reference _div body as _div_cpu, the decimal printer as print_cpu)
+//     are the oracle's restatement of the reference.
+//
+// Usage: binding_check MODE, MODE one of
+//   gpu     the device must be present; every product, division and
+//           decimal output above the thresholds runs on it;
+//   nogpu   no device is expected (nnh_init fails): every call falls back;
+//   inject  the device is present and each operation's first device call
+//           fails by nnh_debug_fail_next(1); the fallback must give the
+//           same results.
+// Prints one line per case and "binding_check ok" at the end; exit 1 on a
+// mismatch.
+#include <algorithm>
+#include <bit>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <string_view>
+#include <utility>
+
+#include "nn_oracle.hpp"
+#include "nnh.h"
+
+constexpr uint64_t NTTmul_threshold = 384;  // ntt_info.h:6
+constexpr uint64_t NTTdiv_threshold = 256;  // ntt_info.h:7
+
+struct basic_natural {
+    uint64_t* data;
+    uint64_t size;
+    uint64_t capacity;
+    basic_natural() : data((uint64_t*)std::malloc(sizeof(uint64_t))), size(1), capacity(1) {
+        if (!data) std::abort();
+    }
+    basic_natural(const basic_natural& n) : size(n.size), capacity(std::bit_ceil(n.size)) {
+        data = (uint64_t*)std::malloc(capacity * sizeof(uint64_t));
+        if (!data) std::abort();
+        std::memcpy(data, n.data, size * sizeof(uint64_t));
+    }
+    basic_natural(basic_natural&& n) noexcept : data(n.data), size(n.size), capacity(n.capacity) { n.data = nullptr; }
+    ~basic_natural() { std::free(data); }
+    basic_natural& operator=(const basic_natural& n) {
+        if (this == &n) return *this;
+        resize(n.size);
+        std::memcpy(data, n.data, size * sizeof(uint64_t));
+        return *this;
+    }
+    basic_natural& operator=(basic_natural&& n) noexcept {
+        if (this == &n) return *this;
+        std::free(data);
+        data = n.data;
+        size = n.size;
+        capacity = n.capacity;
+        n.data = nullptr;
+        return *this;
+    }
+    void resize(uint64_t s) {
+        size = s;
+        if (capacity >= s) return;
+        capacity = std::bit_ceil(s);
+        data = (uint64_t*)std::realloc(data, capacity * sizeof(uint64_t));
+        if (!data) std::abort();
+    }
+};
+
+struct natural : basic_natural {
+    natural() { data[0] = 0; }
+    natural(uint64_t v) { data[0] = v; }
+    natural(const nno::Nat& n) {
+        resize(n.size);
+        std::memcpy(data, n.data, n.size * sizeof(uint64_t));
+    }
+    natural(const natural&) = default;
+    natural(natural&&) noexcept = default;
+    natural& operator=(const natural&) = default;
+    natural& operator=(natural&&) noexcept = default;
+
+    nno::Nat nat() const { return nno::Nat(data, size); }
+    natural& std() {  // natural.h:82-93
+        while (size > 1 && data[size - 1] == 0) size--;
+        return *this;
+    }
+    natural& set(const nno::Nat& n) {
+        natural t(n);
+        *this = std::move(t);
+        return std();
+    }
+    // CPU arms: the oracle's restatement (results land in a fresh natural, so
+    // every aliasing of this with an operand is safe, as in natural.h:356-382)
+    natural& _mul_base(const natural* ap, const natural* bp) { return set(nno::mul(ap->nat(), bp->nat())); }
+    natural& _mul_NTT(const natural* ap, const natural* bp) { return set(nno::mul(ap->nat(), bp->nat())); }
+    natural& _sqr_base(const natural* p) { return set(nno::sqr(p->nat())); }
+    natural& _sqr_NTT(const natural* p) { return set(nno::sqr(p->nat())); }
+    natural& _div_cpu(const natural* ap, const natural* bp, natural* r) {  // natural.h:749-769
+        nno::Nat q, rr;
+        nno::divmod(ap->nat(), bp->nat(), q, rr);
+        r->set(rr);
+        return set(q);
+    }
+
+    natural& _mul(const natural* ap, const natural* bp);
+    natural& _sqr(const natural* p);
+    natural& _div(const natural* ap, const natural* bp, natural* r);
+};
+
+static std::ostream& print_cpu(std::ostream& os, const natural& n) { return os << nno::to_dec(n.nat()); }
+
+// natural.h:537-564, 771-793
+natural& operator*=(natural& a, const natural& b) { return a._mul(&a, &b); }
+natural operator*(const natural& a, const natural& b) {
+    natural d;
+    d._mul(&a, &b);
+    return d;
+}
+natural sqr(const natural& a) {
+    natural d;
+    d._sqr(&a);
+    return d;
+}
+natural& operator/=(natural& a, const natural& b) {
+    natural r;
+    a._div(&a, &b, &r);
+    return a;
+}
+natural operator/(const natural& a, const natural& b) {
+    natural q, r;
+    q._div(&a, &b, &r);
+    return q;
+}
+natural& operator%=(natural& a, const natural& b) {
+    natural q;
+    q._div(&a, &b, &a);
+    return a;
+}
+natural operator%(const natural& a, const natural& b) {
+    natural q, r;
+    q._div(&a, &b, &r);
+    return r;
+}
+
+#include "binding_snippets.inc"
+
+namespace {
+
+int g_mode = 0;  // 0 gpu, 1 nogpu, 2 inject
+int g_bad = 0;
+
+uint64_t sm_next(uint64_t& x) {  // splitmix64
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+natural rnd(uint64_t seed, uint64_t n) {
+    natural x;
+    x.resize(n);
+    for (uint64_t i = 0; i < n; i++) x.data[i] = sm_next(seed);
+    if (!x.data[n - 1]) x.data[n - 1] = 1;
+    return x;
+}
+// Same value (the oracle's quotient may carry a top zero word).
+bool same(const natural& a, const nno::Nat& b0) {
+    nno::Nat b = b0;
+    b.normalize();
+    return a.size == b.size && std::memcmp(a.data, b.data, a.size * sizeof(uint64_t)) == 0;
+}
+// Before each operation in inject mode: make the next device call fail.
+void arm() {
+    if (g_mode == 2) nnh_debug_fail_next(1);
+}
+// After it: in inject mode the injected failure must have been consumed (the
+// binding did call the device and then fell back).
+void disarm(const char* what, bool device_expected) {
+    if (g_mode != 2) return;
+    const bool hit = nnh_debug_fail_next(0) == 0;  // the armed failure was consumed
+    if (device_expected && !hit) {
+        std::printf("FAIL %s: the injected failure was not reached\n", what);
+        g_bad++;
+    }
+}
+void check(const char* what, bool ok) {
+    std::printf("%s %s\n", ok ? "ok  " : "FAIL", what);
+    std::fflush(stdout);
+    if (!ok) g_bad++;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string m = argc > 1 ? argv[1] : "";
+    if (m == "gpu") g_mode = 0;
+    else if (m == "nogpu") g_mode = 1;
+    else if (m == "inject") g_mode = 2;
+    else {
+        std::fprintf(stderr, "usage: binding_check gpu|nogpu|inject\n");
+        return 2;
+    }
+    bool dev = nnh_ready();
+    if (g_mode == 1 && dev) {
+        std::printf("FAIL a device is present in nogpu mode\n");
+        return 1;
+    }
+    if (g_mode != 1 && !dev) {
+        std::printf("FAIL no device: %s\n", nnh_last_error());
+        return 1;
+    }
+    const bool on_dev = g_mode != 1;
+
+    // a *= a (natural.h:357-362 aliasing), above the threshold
+    {
+        natural a = rnd(1, 1000);
+        nno::Nat want = nno::sqr(a.nat());
+        arm();
+        a *= a;
+        disarm("a *= a", on_dev);
+        check("a *= a (1000 words)", same(a, want));
+    }
+    // products across the threshold, both operand orders
+    for (uint64_t n : {383ull, 384ull, 385ull, 2000ull}) {
+        natural a = rnd(10 + n, n), b = rnd(20 + n, 1200);
+        nno::Nat want = nno::mul(a.nat(), b.nat());
+        arm();
+        natural c = a * b;
+        disarm("a * b", on_dev && n >= NTTmul_threshold);
+        arm();
+        a *= b;
+        disarm("a *= b", on_dev && n >= NTTmul_threshold);
+        std::string w = "a * b and a *= b (" + std::to_string(n) + " x 1200 words)";
+        check(w.c_str(), same(c, want) && same(a, want));
+    }
+    // sqr
+    {
+        natural a = rnd(3, 700);
+        nno::Nat want = nno::sqr(a.nat());
+        arm();
+        natural s = sqr(a);
+        disarm("sqr", on_dev);
+        check("sqr(a) (700 words)", same(s, want));
+    }
+    // a %= b: the remainder aliases the dividend (natural.h:783-787)
+    {
+        natural a = rnd(4, 3000), b = rnd(5, 1000);
+        nno::Nat q, r;
+        nno::divmod(a.nat(), b.nat(), q, r);
+        arm();
+        a %= b;
+        disarm("a %= b", on_dev);
+        check("a %= b (3000 / 1000 words)", same(a, r));
+    }
+    // a /= b, then q = a / b and r = a % b as values
+    {
+        natural a = rnd(6, 2500), b = rnd(7, 900);
+        nno::Nat q, r;
+        nno::divmod(a.nat(), b.nat(), q, r);
+        arm();
+        natural q2 = a / b;
+        disarm("a / b", on_dev);
+        arm();
+        natural r2 = a % b;
+        disarm("a % b", on_dev);
+        arm();
+        a /= b;
+        disarm("a /= b", on_dev);
+        check("a / b, a % b, a /= b (2500 / 900 words)", same(q2, q) && same(r2, r) && same(a, q));
+    }
+    // a /= a and a %= a: every pointer the same object
+    {
+        natural a = rnd(8, 1500), b = rnd(8, 1500);
+        arm();
+        a /= a;
+        disarm("a /= a", on_dev);
+        arm();
+        b %= b;
+        disarm("a %= a", on_dev);
+        check("a /= a is 1, a %= a is 0 (1500 words)", a.size == 1 && a.data[0] == 1 && b.size == 1 && b.data[0] == 0);
+    }
+    // one-word divisor above the device threshold
+    {
+        natural a = rnd(9, 800), b(1000000007ull);
+        nno::Nat q, r;
+        nno::divmod(a.nat(), b.nat(), q, r);
+        natural rr;
+        arm();
+        a._div(&a, &b, &rr);
+        disarm("a / word", on_dev);
+        check("a /= one word (800 words)", same(a, q) && same(rr, r));
+    }
+    // decimal output above the device size
+    {
+        natural a = rnd(11, 300);
+        std::string want = nno::to_dec(a.nat());
+        std::ostringstream os;
+        arm();
+        os << a;
+        disarm("operator<<", on_dev);
+        check("operator<< decimal (300 words)", os.str() == want);
+    }
+    if (g_bad) {
+        std::printf("binding_check: %d failures\n", g_bad);
+        return 1;
+    }
+    std::printf("binding_check ok (%s)\n", m.c_str());
+    return 0;
+}
