@@ -115,6 +115,21 @@ int nnh_mul_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an,
                          const uint64_t* d_b, uint64_t bn, void* stream);
 int nnh_sqr_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, void* stream);
 
+/* Allocation-free form of nnh_mul_device_async.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same product, same reference dispatch (schoolbook below 384
+ * words, else the transform with the one-level split); it calls no
+ * hipMalloc/hipFree.  Its workspace and twiddle tables are a fixed array in
+ * device memory (84 MiB per GPU, held from nnh_init until the process
+ * ends), which limits the product to an + bn <= 2^20 words (NNH_ERR_ARG
+ * beyond).  Every call on a GPU shares that array: the library orders calls
+ * it sees one after another like its other calls, but replays of captured
+ * calls that overlap on one GPU race on it.  d_dst must not overlap d_a or
+ * d_b. */
+int nnh_mul_device_noalloc(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
+                           void* stream);
+
 /* Cyclic product (a*b) mod (2^(64*2^scale) - 1) as the reference's
  * NTT_info::save computes it when the product is longer than the transform
  * (ntt_info.h:238-242): dst receives 2^scale words.  Requires

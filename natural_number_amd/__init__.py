@@ -51,6 +51,7 @@ def lib():
         "nnh_sqr": ([_u64p, _u64p, u64], i32),
         "nnh_mul_device_async": ([vp, vp, u64, vp, u64, vp], i32),
         "nnh_sqr_device_async": ([vp, vp, u64, vp], i32),
+        "nnh_mul_device_noalloc": ([vp, vp, u64, vp, u64, vp], i32),
         "nnh_mul_cyclic": ([_u64p, _u64p, u64, _u64p, u64, i32], i32),
         "nnh_transform_forward": ([_u64p, _u64p, u64, i32], i32),
         "nnh_alpha": ([i32, _u64p], i32),
@@ -188,6 +189,11 @@ def mul_transformed(fa, an, b, scale):
 def mul_device_async(d_out, d_a, an, d_b, bn, stream=None):
     """Device-pointer product (pointers as ints), queued on `stream`."""
     _check(lib().nnh_mul_device_async(d_out, d_a, an, d_b, bn, stream))
+
+
+def mul_device_noalloc(d_out, d_a, an, d_b, bn, stream=None):
+    """Allocation-free product (an + bn <= 2^20 words), device pointers."""
+    _check(lib().nnh_mul_device_noalloc(d_out, d_a, an, d_b, bn, stream))
 
 
 def sqr_device_async(d_out, d_a, an, stream=None):

@@ -88,6 +88,23 @@ struct Twiddles {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Allocation-free products (nnh_mul_device_noalloc): products of up to
+// 2^NOALLOC_SCALE words through transforms of at most that length, with the
+// twiddle tables, the transform workspace, the schoolbook columns / split
+// addend and the CRT tile flags carved out of one file-scope device array.
+// Each GPU holds its own copy for the life of the process (84 MiB); its
+// address comes from hipGetSymbolAddress at nnh_init, per device.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_SCALE = 20;
+constexpr u64 NA_TW = 1ull << (NOALLOC_SCALE - 2);       // entries per twiddle table (6 tables)
+constexpr u64 NA_WS = 6ull << NOALLOC_SCALE;             // A and B transforms, 3 primes each
+constexpr u64 NA_TMP = (3ull << NOALLOC_SCALE) + 64;     // schoolbook columns or split addend
+constexpr u64 NA_TILES = 4096;                           // CRT tile carry functions (words)
+constexpr u64 NA_WORDS = 6 * NA_TW + NA_WS + NA_TMP + NA_TILES;
+constexpr int NA_MAX_DEVICES = 64;
+static __device__ __align__(256) u64 scratch[NA_WORDS];
+
 struct Device {
     int id = -1;
     hipStream_t stream = nullptr;
@@ -129,6 +146,8 @@ struct Device {
     Buf stage;                 // host-API staging: a, b, dst
     Buf tmp;                   // split addend / cyclic words / columns
     Buf bws, bjobs, btiles;    // workspace of internal batches (reused across calls)
+    // device address of `scratch` per device id (null until nnh_init on it)
+    u64* na_ptr[NA_MAX_DEVICES]{};
     std::mutex mu;
 };
 
@@ -345,9 +364,62 @@ static int device_init(int dev) {
     }
     g_dev.id = dev;
     build_constants(g_dev);
+    if (dev < NA_MAX_DEVICES && !g_dev.na_ptr[dev]) {
+        // the allocation-free path's twiddle tables, built once per device
+        u64* p = nullptr;
+        HIPCHK(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+        for (int i = 0; i < 3; i++) {
+            launch_twiddles(p + (2 * i) * NA_TW, NA_TW, g_dev.rb, i, 0, g_dev.stream);
+            launch_twiddles(p + (2 * i + 1) * NA_TW, NA_TW, g_dev.rb, i, 1, g_dev.stream);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(g_dev.stream));
+        g_dev.na_ptr[dev] = p;
+    }
     g_inited = true;
     return NNH_OK;
 }
+
+// For the duration of one allocation-free call: the workspace buffers and
+// twiddle tables the product functions read become fixed views of the
+// device's scratch array (ensure() and ensure_twiddles() then check sizes
+// instead of allocating); the previous ones are restored afterwards.
+struct ScratchScope {
+    Device& D;
+    Buf ws, tmp, tiles;
+    const u64* twf[3];
+    const u64* twi[3];
+    int tw_scale;
+    bool prof;
+    ScratchScope(Device& d, u64* base)
+        : D(d), ws(d.ws), tmp(d.tmp), tiles(d.tiles), tw_scale(d.tw_scale), prof(g_prof) {
+        g_prof = false;  // per-kernel timing creates events
+        for (int i = 0; i < 3; i++) {
+            twf[i] = D.twf[i];
+            twi[i] = D.twi[i];
+            D.twf[i] = base + (2 * i) * NA_TW;
+            D.twi[i] = base + (2 * i + 1) * NA_TW;
+        }
+        D.tw_scale = NOALLOC_SCALE;
+        D.tw_fixed = true;
+        u64* w = base + 6 * NA_TW;
+        D.ws = Buf{w, NA_WS, true};
+        D.tmp = Buf{w + NA_WS, NA_TMP, true};
+        D.tiles = Buf{w + NA_WS + NA_TMP, NA_TILES, true};
+    }
+    ~ScratchScope() {
+        D.ws = ws;
+        D.tmp = tmp;
+        D.tiles = tiles;
+        for (int i = 0; i < 3; i++) {
+            D.twf[i] = twf[i];
+            D.twi[i] = twi[i];
+        }
+        D.tw_scale = tw_scale;
+        D.tw_fixed = false;
+        g_prof = prof;
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Transform building blocks (device pointers, queued on st)
@@ -1193,6 +1265,23 @@ int nnh_mul_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, cons
     if (!d_dst || !d_a || !d_b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
     ENTER(stream ? (hipStream_t)stream : g_dev.stream);
     int rc = mul_device(g_dev, d_dst, d_a, an, d_b, bn, st);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    return NNH_OK;
+}
+
+int nnh_mul_device_noalloc(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
+                           void* stream) {
+    if (!d_dst || !d_a || !d_b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
+    if (an + bn > (1ull << NOALLOC_SCALE))
+        return fail(NNH_ERR_ARG, "product of " + std::to_string(an + bn) + " words exceeds the allocation-free limit of 2^" +
+                                     std::to_string(NOALLOC_SCALE) + " words");
+    ENTER(stream ? (hipStream_t)stream : g_dev.stream);
+    Device& D = g_dev;
+    u64* base = D.id >= 0 && D.id < NA_MAX_DEVICES ? D.na_ptr[D.id] : nullptr;
+    if (!base) return fail(NNH_ERR_NODEV, "no scratch array on this device");
+    ScratchScope sc(D, base);
+    int rc = mul_device(D, d_dst, d_a, an, d_b, bn, st);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
     return NNH_OK;
