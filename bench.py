@@ -200,9 +200,13 @@ def batch_cpu_baseline(ops, sizes, gpu_out=None, stride=4, max_threads=16):
 
 
 def batch_mix(nn, stream, valu, reps=5, with_cpu=True):
-    """Config 3: 4096 uneven products; batched (bucketed kernels,
-    one hipGraph) against one launch sequence per product on the same stream;
-    with_cpu: the oracle's products/s on the same mix beside them."""
+    """Config 3: 4096 uneven products, batched (bucketed kernels, one
+    hipGraph) with each request's arm picked by the installed cost-model
+    policy, and with the reference's rule (schoolbook below 384 words, one
+    transform above), against the plain path of one launch sequence per
+    product through the reference dispatch (NNH_ALGO_REFERENCE) on the same
+    stream; with_cpu: the oracle's products/s on the same mix beside them,
+    every product compared with the batch's."""
     from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
 
     log("batch mix: generating operands")
@@ -211,56 +215,103 @@ def batch_mix(nn, stream, valu, reps=5, with_cpu=True):
     host = np.concatenate([np.concatenate([a, b]) for a, b in ops])
     inp = nn.DeviceWords(host.size, host)
     n_out = sum(a + b for a, b in sizes)
-    out_b = nn.DeviceWords(n_out)
-    out_s = nn.DeviceWords(n_out)
-    reqs_b, reqs_s = [], []
+    outs = {k: nn.DeviceWords(n_out) for k in ("policy", "reference", "single")}
+    reqs = {k: [] for k in outs}
     ia = oa = 0
     for an, bn in sizes:
         da, db = inp.at(ia), inp.at(ia + an)
-        reqs_b.append((out_b.at(oa), da, an, db, bn))
-        reqs_s.append((out_s.at(oa), da, an, db, bn))
+        for k in outs:
+            reqs[k].append((outs[k].at(oa), da, an, db, bn))
         ia += an + bn
         oa += an + bn
     sp = stream.ptr
+    policy = nn.policy_active()
+    arms = [nn.policy_arm(an, bn, 64)[0] for an, bn in sizes] if policy else []
     log("batch mix: planning + graph capture")
-    b = nn.Batch(reqs_b, graph=True)
-    b.run(sp)
-    for r in reqs_s:
-        nn.mul_device_async(*r, sp)
+    bp = nn.Batch(reqs["policy"], graph=True)
+    br = nn.Batch(reqs["reference"], graph=True, reference=True)
+    bp.run(sp)
+    br.run(sp)
+    for r in reqs["single"]:
+        nn.mul_algo_device_async(*r, nn.ALGO_REFERENCE, 0, sp)
     nn.synchronize()
-    host_b = out_b.download()
-    exact = bool(np.array_equal(host_b, out_s.download()))
+    host_b = outs["policy"].download()
+    exact_ref = bool(np.array_equal(host_b, outs["reference"].download()))
+    exact_single = bool(np.array_equal(host_b, outs["single"].download()))
     e0, e1 = nn.Event(), nn.Event()
-    nn.synchronize()
-    e0.record(stream)
-    for _ in range(reps):
-        b.run(sp)
-    e1.record(stream)
-    tb = e0.elapsed_ms(e1) / reps
-    e0.record(stream)
-    for _ in range(2):
-        for r in reqs_s:
-            nn.mul_device_async(*r, sp)
-    e1.record(stream)
-    ts = e0.elapsed_ms(e1) / 2
-    b.close()
-    for x in (inp, out_b, out_s):
+
+    def timed(fn, n):
+        nn.synchronize()
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        return e0.elapsed_ms(e1) / n
+
+    tb = timed(lambda: bp.run(sp), reps)
+    tr = timed(lambda: br.run(sp), reps)
+
+    def singles():
+        for r in reqs["single"]:
+            nn.mul_algo_device_async(*r, nn.ALGO_REFERENCE, 0, sp)
+
+    ts = timed(singles, 2)
+    bp.close()
+    br.close()
+    inp.free()
+    for x in outs.values():
         x.free()
     n = len(sizes)
     ideal = batch_ideal_ms(sizes, valu["peak_lazy_bfly_per_s"], valu["peak_strict_bfly_per_s"])
+    arm_counts = {a: arms.count(i) for i, a in enumerate(["school", "karatsuba", "ntt", "ntt_split", "ntt_chunk",
+                                                           "ntt_chunk+1"])} if policy else None
+    if policy:
+        arm_counts["reference_rule"] = arms.count(-1)
     return {
         "workload": "4096 products, operand bits log-uniform in [2^12, 2^24], half balanced, half up to 64:1 "
                     "(generator seed 0x4E4E2E1972800003, splitmix64 operands)",
         "products": n,
+        "cost_model_policy": policy,
+        "policy_arms": arm_counts,
         "batched_ms": tb,
         "batched_products_per_s": n / (tb * 1e-3),
+        "batched_reference_rule_ms": tr,
+        "batched_reference_rule_products_per_s": n / (tr * 1e-3),
+        "policy_gain_over_reference_rule": tr / tb,
         "per_product_launch_ms": ts,
         "per_product_launch_products_per_s": n / (ts * 1e-3),
         "speedup": ts / tb,
-        "bit_exact_batched_vs_single": exact,
-        "valu_ceiling": {"ideal_ms": round(ideal, 3), "frac": round(ideal / tb, 3)},
+        "bit_exact_batched_vs_reference_rule": exact_ref,
+        "bit_exact_batched_vs_single": exact_single,
+        "valu_ceiling": {"ideal_ms": round(ideal, 3), "frac": round(ideal / tb, 3),
+                         "note": "ideal counts one transform per NTT request (the reference rule); the policy "
+                                 "runs fewer butterflies on unbalanced requests"},
         "cpu_baseline": batch_cpu_baseline(ops, sizes, host_b) if with_cpu else None,
     }
+
+
+def policy_eval(nn, n_held):
+    """Config 5 on the installed policy: the shipped network (policy file
+    beside the library) against exhaustive autotune on n_held held-out
+    log-uniform requests of 2^10 .. 2^30 bits (generator seed
+    0x4E4E2E1972800005, the first n_held of the 10^4 stream)."""
+    from natural_number_amd import costmodel as cm
+    from natural_number_amd.workloads import log_uniform_requests
+
+    path = os.path.join(ROOT, "natural_number_amd", "policy_gfx950.txt")
+    if not os.path.exists(path):
+        return None
+    log(f"cost model: evaluating the shipped policy on {n_held} held-out requests")
+    model = cm.CostModel(seed=7)
+    model.set_params(cm.load_policy_params(path))
+    bench = cm.Bench()
+    t = time.perf_counter()
+    ev = cm.evaluate(bench, model, log_uniform_requests(10000, 0x4E4E2E1972800005)[:n_held], log=log)
+    ev["seconds_evaluate"] = round(time.perf_counter() - t, 1)
+    ev["policy_file"] = "natural_number_amd/policy_gfx950.txt"
+    bench.close()
+    model.close()
+    return ev
 
 
 def e_workload(nn, with_cpu):
@@ -363,11 +414,14 @@ def drivers_workload(nn, with_cpu):
     return res
 
 
-def cost_model_run(n_side, n_held):
+def cost_model_run(n_side, n_held, save_policy=None):
     """Config 5: timings over a log-uniform (m, n) grid of 2^10 .. 2^30-bit
     operands and batch counts, the FP32 MLP trained on the GPU by the
-    library's kernels, and its regret against exhaustive autotune on held-out
-    log-uniform requests (generator seed 0x4E4E2E1972800005)."""
+    library's kernels, installed as the library's dispatch policy, and its
+    regret against exhaustive autotune on held-out log-uniform requests
+    (generator seed 0x4E4E2E1972800005).  save_policy: write the trained
+    network as a policy file (the library loads policy_gfx950.txt beside
+    itself at nnh_init)."""
     import natural_number_amd as nn
     from natural_number_amd import costmodel as cm
     from natural_number_amd.workloads import log_uniform_requests
@@ -387,6 +441,8 @@ def cost_model_run(n_side, n_held):
     for lr in (3e-3, 1e-3, 3e-4):
         loss = model.train(X, T, M, steps=4000, lr=lr)
     t_train = time.perf_counter() - t1
+    if save_policy:
+        cm.save_policy(model, save_policy)
     reqs = log_uniform_requests(n_held, 0x4E4E2E1972800005)
     log(f"cost model: evaluating {len(reqs)} held-out requests")
     t2 = time.perf_counter()
@@ -464,15 +520,20 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-mix measurement")
     ap.add_argument("--no-e", action="store_true", help="skip the e-digits workload")
     ap.add_argument("--no-drivers", action="store_true", help="skip the pi and factorial workloads")
+    ap.add_argument("--no-policy", action="store_true", help="skip the cost-model policy evaluation")
     ap.add_argument("--cost-model", action="store_true",
                     help="only run the cost-model pipeline (grid timings, on-device training, held-out regret)")
     ap.add_argument("--held-out", type=int, default=10000, help="held-out requests for --cost-model")
     ap.add_argument("--grid", type=int, default=28, help="grid points per side for --cost-model")
+    ap.add_argument("--save-policy", default=None, metavar="PATH",
+                    help="with --cost-model: write the trained network as a policy file")
+    ap.add_argument("--policy-held-out", type=int, default=2000,
+                    help="held-out requests on which the default run evaluates the installed policy")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.cost_model:
-        print(json.dumps(cost_model_run(args.grid, args.held_out)), flush=True)
+        print(json.dumps(cost_model_run(args.grid, args.held_out, args.save_policy)), flush=True)
         return
     if args.copy_calibration:
         import natural_number_amd as nn
@@ -556,6 +617,7 @@ def main():
 
     c2 = c2_variants(nn, stream, max(3, args.steps // 4), not args.no_cpu)
     batch = None if args.no_batch else batch_mix(nn, stream, valu, with_cpu=not args.no_cpu)
+    cmodel = None if args.no_policy else policy_eval(nn, args.policy_held_out)
     e_res = None if args.no_e else e_workload(nn, not args.no_cpu)
     drv = None if args.no_drivers else drivers_workload(nn, not args.no_cpu)
 
@@ -617,6 +679,7 @@ def main():
         "cpu_baseline": cpu,
         "c2_variants": c2,
         "batch_mix": batch,
+        "cost_model": cmodel,
         "e_digits": e_res,
         "binary_splitting_drivers": drv,
         "bit_exact_vs_cpu": exact,

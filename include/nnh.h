@@ -67,13 +67,17 @@ int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an);
  * exact; `param` is the piece transform scale for NNH_ALGO_NTT_CHUNK
  * (0 = smallest that holds the shorter operand twice).  Used by the
  * autotuner and the cost model. */
-#define NNH_ALGO_AUTO 0        /* reference dispatch: schoolbook below 384 words, else NTT with one split */
+#define NNH_ALGO_AUTO 0        /* the library's dispatch: the installed cost-model policy (nnh_policy_*),
+                                  else the reference's (schoolbook below 384 words, else NTT with one split) */
 #define NNH_ALGO_SCHOOL 1      /* column-wise schoolbook (asmMul1) */
 #define NNH_ALGO_KARATSUBA 2   /* one Karatsuba level over schoolbook columns (toom22mul) */
 #define NNH_ALGO_NTT 3         /* one transform for the whole product */
 #define NNH_ALGO_NTT_SPLIT 4   /* the reference's one-level unbalanced split (_mul_NTT) */
 #define NNH_ALGO_NTT_CHUNK 5   /* larger operand in pieces, all pieces one batch sharing the shorter
                                   operand's transform; synchronises the stream */
+#define NNH_ALGO_REFERENCE 6   /* the reference's fixed thresholds, whatever policy is installed */
+#define NNH_ALGO_MODEL 7       /* the installed cost-model policy (NNH_ERR_ARG if none); param = count of
+                                  back-to-back products the choice assumes (0 = 1) */
 int nnh_mul_algo(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, uint64_t bn, int algo, int param);
 int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn,
                               int algo, int param, void* stream);
@@ -106,6 +110,33 @@ int nnh_mlp_train(nnh_mlp* m, const float* x, const float* target, const float* 
                   float lr, float* final_loss);
 /* Index of the smallest predicted output among those with feasible > 0. */
 int nnh_mlp_choose(nnh_mlp* m, const float* x, const float* feasible, uint64_t n, int* choice);
+
+/* The cost model as the library's dispatch policy (replaces the fixed
+ * thresholds of natural::_mul, natural.h:444-450, ntt_info.h:6-8, base.h:18,37;
+ * no counterpart in the reference).  Arms, in the network's output order:
+ * 0 schoolbook, 1 Karatsuba, 2 one transform, 3 the reference split,
+ * 4 chunked at the smallest piece scale holding the shorter operand twice,
+ * 5 chunked one scale up.  nnh_policy_set runs the network [3 -> 64 -> 64 -> 6]
+ * on the device over a table of operand sizes (1/16 octave, 2^10 .. 2^30 bits
+ * each, 1/8/64 back-to-back products) and installs the predictions: from then
+ * on nnh_mul, nnh_mul_device_async, the natural-layer calls and batches pick
+ * per request the arm with the smallest predicted time among those feasible
+ * at its exact sizes; sizes outside the table keep the reference's rule.
+ * Every arm is exact, so results do not change, only time.  NULL removes the
+ * policy.  nnh_policy_load reads a file ("nnh-policy 1 3 6 <n>" then n
+ * parameters); at nnh_init the library installs $NNH_POLICY, or the
+ * policy_gfx950.txt beside it when present (NNH_POLICY=reference: none). */
+int nnh_policy_set(nnh_mlp* m);
+int nnh_policy_load(const char* path);
+int nnh_policy_active(void);
+/* The arm the dispatch takes for an an x bn product among `count`
+ * back-to-back ones: arm = -1 (and algo NNH_ALGO_REFERENCE) when no policy
+ * applies.  algo/param are those of nnh_mul_algo. */
+int nnh_policy_arm(uint64_t an, uint64_t bn, uint64_t count, int* arm, int* algo, int* param);
+/* feasible[0..5] = 1 for the arms that are candidates at these sizes. */
+int nnh_arm_feasible(uint64_t an, uint64_t bn, float* feasible);
+/* (algo, param) of nnh_mul_algo that runs arm `arm` at these sizes. */
+int nnh_arm_call(int arm, uint64_t an, uint64_t bn, int* algo, int* param);
 
 /* Device-pointer variants: operands and result already in HBM, work is
  * queued on `stream` (a hipStream_t, NULL = the library's stream) and the
@@ -170,10 +201,17 @@ typedef struct nnh_req {
 
 typedef struct nnh_batch nnh_batch;
 
-/* Plan a batch: buckets, workspace, device job table.  flags bit 0: capture
- * the launch sequence into a hipGraph replayed by nnh_batch_run.  The plan
- * keeps the request pointers: rerunning recomputes from their current
- * contents. */
+/* Plan a batch: buckets, workspace, device job table.  Flags:
+ * NNH_BATCH_GRAPH captures the launch sequence into a hipGraph replayed by
+ * nnh_batch_run; with a cost-model policy installed each request runs the
+ * arm it picks (schoolbook, one transform, or the chunked transform whose
+ * pieces join the buckets and share the shorter operand's transform; the
+ * reference split runs as the chunked form at its scale), unless
+ * NNH_BATCH_REFERENCE asks for the reference's rule (schoolbook below 384
+ * words, else one transform).  The plan keeps the request pointers:
+ * rerunning recomputes from their current contents. */
+#define NNH_BATCH_GRAPH 1
+#define NNH_BATCH_REFERENCE 8
 int nnh_batch_create(const nnh_req* reqs, uint64_t n, int flags, nnh_batch** out);
 /* Queue one execution of the batch on stream (NULL = library stream). */
 int nnh_batch_run(nnh_batch* b, void* stream);

@@ -79,6 +79,12 @@ def lib():
         "nnh_mlp_gradients": ([vp, vp, vp, vp, u64, vp, ctypes.POINTER(ctypes.c_float)], i32),
         "nnh_mlp_train": ([vp, vp, vp, vp, u64, i32, ctypes.c_float, ctypes.POINTER(ctypes.c_float)], i32),
         "nnh_mlp_choose": ([vp, vp, vp, u64, vp], i32),
+        "nnh_policy_set": ([vp], i32),
+        "nnh_policy_load": ([ctypes.c_char_p], i32),
+        "nnh_policy_active": ([], i32),
+        "nnh_policy_arm": ([u64, u64, u64, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
+        "nnh_arm_feasible": ([u64, u64, ctypes.POINTER(ctypes.c_float)], i32),
+        "nnh_arm_call": ([i32, u64, u64, ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
         "nnh_device_alloc": ([ctypes.POINTER(vp), u64], i32),
         "nnh_device_free": ([vp], i32),
         "nnh_copy_to_device": ([vp, vp, u64], i32),
@@ -132,8 +138,9 @@ def mul(a, b):
     return out
 
 
-ALGO_AUTO, ALGO_SCHOOL, ALGO_KARATSUBA, ALGO_NTT, ALGO_NTT_SPLIT, ALGO_NTT_CHUNK = range(6)
-ALGO_NAMES = ["auto", "school", "karatsuba", "ntt", "ntt_split", "ntt_chunk"]
+ALGO_AUTO, ALGO_SCHOOL, ALGO_KARATSUBA, ALGO_NTT, ALGO_NTT_SPLIT, ALGO_NTT_CHUNK, ALGO_REFERENCE, ALGO_MODEL = range(8)
+ALGO_NAMES = ["auto", "school", "karatsuba", "ntt", "ntt_split", "ntt_chunk", "reference", "model"]
+BATCH_GRAPH, BATCH_REFERENCE = 1, 8
 
 
 def mul_algo(a, b, algo, param=0):
@@ -232,10 +239,13 @@ def kernel_times():
 class Batch:
     """A planned batch of device-pointer products (nnh_batch_create)."""
 
-    def __init__(self, reqs, graph=True):
+    def __init__(self, reqs, graph=True, reference=False):
+        """graph: capture into a hipGraph; reference: plan with the
+        reference's rule even when a cost-model policy is installed."""
         arr = (Req * len(reqs))(*[Req(*r) for r in reqs])
         h = ctypes.c_void_p()
-        _check(lib().nnh_batch_create(arr, len(reqs), 1 if graph else 0, ctypes.byref(h)))
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_REFERENCE if reference else 0)
+        _check(lib().nnh_batch_create(arr, len(reqs), flags, ctypes.byref(h)))
         self.h = h
 
     def run(self, stream=None):
@@ -251,6 +261,26 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def policy_active():
+    """True when a cost-model policy drives the library's dispatch."""
+    return bool(lib().nnh_policy_active())
+
+
+def policy_load(path):
+    _check(lib().nnh_policy_load(str(path).encode()))
+
+
+def policy_clear():
+    _check(lib().nnh_policy_set(None))
+
+
+def policy_arm(an, bn, count=1):
+    """(arm, algo, param) the dispatch takes; arm -1 = the reference's rule."""
+    arm, algo, param = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib().nnh_policy_arm(an, bn, count, ctypes.byref(arm), ctypes.byref(algo), ctypes.byref(param)))
+    return arm.value, algo.value, param.value
 
 
 def mul_batch(reqs, stream=None):

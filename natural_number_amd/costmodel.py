@@ -7,16 +7,20 @@ piece scales.  The autotuner times every feasible arm on the GPU
 (nnh_time_algo, a C-side loop of back-to-back launches); the cost model is
 an FP32 MLP [log2 bits of the larger operand, log2 bits of the smaller,
 log2 batch count] -> predicted log2 milliseconds per arm, trained and run by
-the library's own HIP kernels (nnh_mlp_*).  The reference's own policy
-(fixed thresholds, natural.h:444-450) is arm NNH_ALGO_AUTO.
+the library's own HIP kernels (nnh_mlp_*).  Installed with
+CostModel.install() (nnh_policy_set) or saved as a policy file
+(save_policy) that the library loads at nnh_init, the model is the
+library's dispatch policy: NNH_ALGO_AUTO, nnh_mul, the natural layer and
+batches then take the arm it picks.  The reference's own policy (fixed
+thresholds, natural.h:444-450) stays available as NNH_ALGO_REFERENCE.
 """
 import ctypes
 import math
 
 import numpy as np
 
-from . import (ALGO_AUTO, ALGO_KARATSUBA, ALGO_NTT, ALGO_NTT_CHUNK, ALGO_NTT_SPLIT, ALGO_SCHOOL, DeviceWords,
-               _check, lib)
+from . import (ALGO_KARATSUBA, ALGO_NTT, ALGO_NTT_CHUNK, ALGO_NTT_SPLIT, ALGO_REFERENCE, ALGO_SCHOOL, DeviceWords,
+               _check, lib, policy_arm)
 
 ARMS = ["school", "karatsuba", "ntt", "ntt_split", "ntt_chunk", "ntt_chunk+1"]
 N_ARMS = len(ARMS)
@@ -53,6 +57,19 @@ def feasible(an, bn):
         pieces = -(-big // piece)
         f[k] = 1.0 if 2 <= pieces <= MAX_PIECES else 0.0
     return f
+
+
+def library_feasible(an, bn):
+    """The library's feasibility rule (nnh_arm_feasible; needs no device)."""
+    f = (ctypes.c_float * N_ARMS)()
+    _check(lib().nnh_arm_feasible(an, bn, f))
+    return np.array(list(f), dtype=np.float32)
+
+
+def library_arm_call(arm, an, bn):
+    algo, param = ctypes.c_int(), ctypes.c_int()
+    _check(lib().nnh_arm_call(arm, an, bn, ctypes.byref(algo), ctypes.byref(param)))
+    return algo.value, param.value
 
 
 def features(an, bn, count=1):
@@ -168,10 +185,36 @@ class CostModel:
         _check(lib().nnh_mlp_choose(self.h, X.ctypes.data, F.ctypes.data, X.shape[0], ch.ctypes.data))
         return ch
 
+    def install(self):
+        """Make this network the library's dispatch policy (nnh_policy_set)."""
+        _check(lib().nnh_policy_set(self.h))
+
     def close(self):
         if self.h:
             lib().nnh_mlp_destroy(self.h)
             self.h = None
+
+
+POLICY_HEADER = "nnh-policy 1 {n_in} {n_out} {n}"
+
+
+def save_policy(model, path):
+    """Write the network as a policy file (nnh_policy_load's format)."""
+    p = model.params()
+    with open(path, "w") as f:
+        f.write(POLICY_HEADER.format(n_in=N_IN, n_out=N_ARMS, n=p.size) + "\n")
+        for k in range(0, p.size, 8):
+            f.write(" ".join(f"{v:.9g}" for v in p[k:k + 8]) + "\n")
+
+
+def load_policy_params(path):
+    with open(path) as f:
+        head = f.readline().split()
+        assert head[:2] == ["nnh-policy", "1"], "not a policy file"
+        n = int(head[4])
+        p = np.array(f.read().split(), dtype=np.float32)
+    assert p.size == n
+    return p
 
 
 def reference_forward(p, X, n_in=N_IN, n_out=N_ARMS, H=64):
@@ -195,11 +238,14 @@ def masked_mse(p, X, T, M):
 
 def evaluate(bench, model, requests, log=None):
     """Exhaustive autotune of every held-out request (the judge) against the
-    model's pick and the reference policy (NNH_ALGO_AUTO)."""
+    arm the library's dispatch takes with the model installed as its policy
+    (nnh_policy_arm), the model evaluated exactly per request on the device
+    (nnh_mlp_choose), and the reference policy (NNH_ALGO_REFERENCE)."""
+    model.install()
     X = np.stack([features(a, b, 1) for a, b in requests])
     F = np.stack([feasible(a, b) for a, b in requests])
-    pick = model.choose(X, F)
-    t_best = t_model = t_auto = 0.0
+    exact = model.choose(X, F)
+    t_best = t_policy = t_exact = t_ref = 0.0
     regrets = []
     hits = 0
     for k, (an, bn) in enumerate(requests):
@@ -208,25 +254,30 @@ def evaluate(bench, model, requests, log=None):
             if F[k, arm]:
                 times[arm] = bench.time_arm(arm, an, bn)
         best = min(times.values())
-        tm = times[int(pick[k])]
-        ta = bench.time(ALGO_AUTO, 0, an, bn)
+        arm, _, _ = policy_arm(an, bn, 1)
+        # sizes outside the policy table take the reference's rule
+        tp = times[arm] if arm >= 0 else bench.time(ALGO_REFERENCE, 0, an, bn)
+        ta = bench.time(ALGO_REFERENCE, 0, an, bn)
         t_best += best
-        t_model += tm
-        t_auto += ta
-        regrets.append(tm / best - 1.0)
-        hits += tm == best
+        t_policy += tp
+        t_exact += times[int(exact[k])]
+        t_ref += ta
+        regrets.append(tp / best - 1.0)
+        hits += tp == best
         if log and k % 500 == 0:
             log(f"evaluate {k}/{len(requests)}")
     r = np.array(regrets)
     return {
         "requests": len(requests),
         "total_ms_exhaustive_best": t_best,
-        "total_ms_model": t_model,
-        "total_ms_reference_policy": t_auto,
-        "regret_total": t_model / t_best - 1.0,
+        "total_ms_model": t_policy,
+        "total_ms_model_exact_inference": t_exact,
+        "total_ms_reference_policy": t_ref,
+        "regret_total": t_policy / t_best - 1.0,
+        "regret_total_exact_inference": t_exact / t_best - 1.0,
         "regret_mean": float(r.mean()),
         "regret_p90": float(np.quantile(r, 0.9)),
         "picked_best_fraction": hits / len(requests),
-        "gain_over_reference_policy": t_auto / t_model,
-        "reference_policy_regret_total": t_auto / t_best - 1.0,
+        "gain_over_reference_policy": t_ref / t_policy,
+        "reference_policy_regret_total": t_ref / t_best - 1.0,
     }

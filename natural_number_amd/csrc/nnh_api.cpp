@@ -10,9 +10,12 @@
 //   -> inverse top passes -> Garner CRT + carry scan.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <atomic>
 #include <bit>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -25,6 +28,7 @@
 #include "nnh_host.hpp"
 #include "nnh_internal.hpp"
 #include "nnh_nat.hpp"
+#include "nnh_plan.hpp"
 
 namespace nnh {
 
@@ -35,11 +39,8 @@ constexpr u64 P[3] = {27ull << 56 | 1, 58ull << 56 | 1, 87ull << 56 | 1};
 // reference's root_base table (ntt_info.h:89, 98, 107); every other entry of
 // that table is a repeated square of it.
 constexpr u64 ROOT_GEN[3] = {11, 2, 310};
-constexpr int MIN_MID_BITS = 8;       // smallest fused-chunk transform (2^8 words)
 constexpr int TILE_BITS = 12;         // top-pass tile and largest fused chunk: 4096 words
 constexpr int MIN_CHUNK_BITS = 10;    // smallest fused chunk beside top passes (64 threads)
-constexpr int MAX_SCALE = 30;         // largest transform handled (2^30 words per prime)
-constexpr u64 NTT_MUL_THRESHOLD = 384;  // reference NTTmul_threshold (ntt_info.h:6)
 
 static u64 mulmod(u64 a, u64 b, u64 p) { return (u64)((u128)a * b % p); }
 static u64 powmod(u64 b, u64 e, u64 p) {
@@ -49,7 +50,6 @@ static u64 powmod(u64 b, u64 e, u64 p) {
     return r;
 }
 static u64 to_mont(u64 x, u64 p) { return (u64)(((u128)(x % p) << 64) % p); }
-static int ntt_scale(u64 s1, u64 s2) { return 64 - std::countl_zero(s1 + s2 - 2); }
 
 static thread_local std::string g_err;
 static int fail(int code, const std::string& msg) {
@@ -825,12 +825,31 @@ static int ntt_refsplit(Device& D, u64* out, const u64* a, u64 an, const u64* b,
     return product_core(D, pl, a + a0n, an - a0n, b, bn, false, fb, out + a0n, an - a0n + bn, D.tmp.p, bn, st);
 }
 
-enum Algo { ALGO_AUTO = 0, ALGO_SCHOOL = 1, ALGO_KARATSUBA = 2, ALGO_NTT = 3, ALGO_NTT_SPLIT = 4, ALGO_NTT_CHUNK = 5 };
+// The installed cost-model policy (nnh_policy_set); read under the device lock.
+static PolicyTable g_policy;
+
+static int mul_algo_device(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, int algo, int param,
+                           hipStream_t st);
+
+// The library's dispatch for one product among `count` back-to-back ones:
+// the arm the installed policy picks, else the reference's thresholds.
+static int mul_dispatch(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, u64 count,
+                        hipStream_t st) {
+    const int arm = g_policy.choose(an, bn, count);
+    if (arm < 0) return mul_device(D, out, a, an, b, bn, st);
+    int algo, param;
+    arm_call(arm, an, bn, &algo, &param);
+    return mul_algo_device(D, out, a, an, b, bn, algo, param, st);
+}
 
 static int mul_algo_device(Device& D, u64* out, const u64* a, u64 an, const u64* b, u64 bn, int algo, int param,
                            hipStream_t st) {
     switch (algo) {
-        case ALGO_AUTO: return mul_device(D, out, a, an, b, bn, st);
+        case ALGO_AUTO: return mul_dispatch(D, out, a, an, b, bn, 1, st);
+        case ALGO_REFERENCE: return mul_device(D, out, a, an, b, bn, st);
+        case ALGO_MODEL:
+            if (!g_policy.on) return fail(NNH_ERR_ARG, "no cost-model policy installed");
+            return mul_dispatch(D, out, a, an, b, bn, std::max(param, 1), st);
         case ALGO_SCHOOL: return an >= bn ? school(D, a, an, b, bn, out, st) : school(D, b, bn, a, an, out, st);
         case ALGO_KARATSUBA: return kara_device(D, out, a, an, b, bn, st);
         case ALGO_NTT: return ntt_direct(D, out, a, an, b, bn, st);
@@ -856,27 +875,16 @@ static int stage_host(Device& D, u64 words) { return ensure(D.stage, words); }
 // Batched products: jobs bucketed by transform scale, one device job table,
 // one workspace; enqueued directly or captured once into a hipGraph.
 // ---------------------------------------------------------------------------
-struct Bucket {
-    int s;
-    u32 first, count;
-    u32 own_first, own_count;  // forward-only entries of the bucket's shared B transforms
-};
-
 }  // namespace nnh
 
 struct nnh_batch {
-    std::vector<nnh::BJob> jobs;  // the requests' jobs, then the forward-only entries
-    nnh::u32 n_req_jobs = 0;
-    std::vector<nnh::Bucket> buckets;
+    nnh::BatchPlan plan;
     nnh::BJob* d_jobs = nullptr;
     nnh::u32* d_sj = nullptr;
-    nnh::u32 n_school = 0;
-    nnh::u64 ncols = 0;
-    nnh::u64 ntiles = 0;
     nnh::u8* d_tiles = nullptr;
+    nnh::AddJob* d_adds = nullptr;  // chunked requests: odd pieces added into dst
+    nnh::u8* d_add_tiles = nullptr;
     nnh::u64* ws = nullptr;
-    nnh::u64 ws_words = 0;
-    int max_s = 0;
     // twiddle tables read by the batch's launches (and baked into its graph),
     // kept alive by tw for as long as the batch exists
     std::shared_ptr<nnh::Twiddles> tw;
@@ -897,17 +905,17 @@ static void batch_free(nnh_batch* B) {
     if (B->graph) (void)hipGraphDestroy(B->graph);
     if (B->cap_stream) (void)hipStreamDestroy(B->cap_stream);
     if (!B->borrowed) {
-        if (B->d_jobs) (void)hipFree(B->d_jobs);
-        if (B->d_sj) (void)hipFree(B->d_sj);
-        if (B->d_tiles) (void)hipFree(B->d_tiles);
-        if (B->ws) (void)hipFree(B->ws);
+        for (void* p : {(void*)B->d_jobs, (void*)B->d_sj, (void*)B->d_tiles, (void*)B->d_adds, (void*)B->d_add_tiles,
+                        (void*)B->ws})
+            if (p) (void)hipFree(p);
     }
     delete B;
 }
 
 // Enqueue every launch of the batch on st.
 static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
-    const bool fan = D.two_streams && D.nwork > 0 && B->buckets.size() > 0;
+    const BatchPlan& P = B->plan;
+    const bool fan = D.two_streams && D.nwork > 0 && P.buckets.size() > 0;
     if (fan) {
         HIPCHK(hipEventRecord(D.ev_bfork, st));
         for (int w = 0; w < D.nwork; w++) HIPCHK(hipStreamWaitEvent(D.work[w], D.ev_bfork, 0));
@@ -918,8 +926,8 @@ static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
         const int l = lane++ % (D.nwork + 1);
         return (fan && l) ? D.work[l - 1] : st;
     };
-    if (B->n_school) launch_school_batch(B->d_jobs, B->d_sj, B->n_school, B->ncols, st);
-    for (const Bucket& bk : B->buckets) {
+    if (P.n_school) launch_school_batch(B->d_jobs, B->d_sj, P.n_school, P.ncols, st);
+    for (const Bucket& bk : P.buckets) {
         const hipStream_t sa = next_stream(), sb = next_stream();
         Plan pl = make_plan(D, bk.s);
         const BJob* jobs = B->d_jobs + bk.first;
@@ -957,119 +965,25 @@ static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
             HIPCHK(hipEventRecord(D.ev_bjoin[w], D.work[w]));
             HIPCHK(hipStreamWaitEvent(st, D.ev_bjoin[w], 0));
         }
-    launch_crt_batch(B->d_jobs, B->n_req_jobs, B->ntiles, B->d_tiles, B->d_tiles + B->ntiles, D.crt, st);
+    launch_crt_batch(B->d_jobs, P.n_req_jobs, P.ntiles, B->d_tiles, B->d_tiles + P.ntiles, D.crt, st);
+    if (!P.adds.empty()) launch_add_batch(B->d_adds, (u32)P.adds.size(), P.add_tiles, B->d_add_tiles, st);
     return NNH_OK;
 }
 
+// flags bit 0: capture into a hipGraph; bit 1: internal batch (buffers from
+// the device's reusable batch workspace; no policy); bit 2: plan each
+// request's arm with the installed policy.
 static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch** out) {
     nnh_batch* B = new nnh_batch();
-    struct Item {
-        u64 idx;
-        int s;  // 0 = schoolbook
-    };
-    std::vector<Item> items;
-    items.reserve(n);
-    for (u64 i = 0; i < n; i++) {
-        const IReq& r = reqs[i];
-        if (!r.dst || !r.a || !r.b || !r.an || !r.bn || !r.n_out || r.n_out > r.an + r.bn + 1 ||
-            (r.n_add && !r.add) || r.n_add > r.n_out) {
-            delete B;
-            return fail(NNH_ERR_ARG, "batch request " + std::to_string(i) + ": null pointer or bad size");
-        }
-        u64 mn = std::min(r.an, r.bn);
-        int s = mn < NTT_MUL_THRESHOLD ? 0 : ntt_scale(r.an, r.bn);
-        if (s > MAX_SCALE) {
-            delete B;
-            return fail(NNH_ERR_ARG, "batch request too large");
-        }
-        items.push_back({i, s});
-        B->max_s = std::max(B->max_s, s);
+    BatchPlan& P = B->plan;
+    std::string err;
+    const bool use_policy = (flags & 4) && !(flags & 2) && g_policy.on;
+    if (plan_batch(reqs, n, use_policy ? &g_policy : nullptr, std::min<u64>(n, 64), P, err)) {
+        delete B;
+        return fail(NNH_ERR_ARG, err);
     }
-    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.s < y.s; });
-    // workspace offsets (pointers hold word offsets into the workspace until rebased below)
-    u64 ws = 0, tiles = 0, cols = 0;
-    // B operands repeated within a bucket (same pointer and size) are
-    // transformed once: the first such job owns the transform, later ones read it
-    std::map<std::pair<const u64*, u64>, u32> owner_of;
-    for (const Item& it : items) {
-        const IReq& r = reqs[it.idx];
-        BJob J{};
-        bool sq = r.a == r.b && r.an == r.bn;
-        J.a = r.a;
-        J.b = r.b;
-        J.an = r.an;
-        J.bn = r.bn;
-        J.out = r.dst;
-        J.n_out = r.n_out;
-        J.add = r.add;
-        J.n_add = r.n_add;
-        J.tile0 = tiles;
-        tiles += crt_tiles(J.n_out);
-        if (it.s == 0) {
-            J.school = 1;
-            J.n_res = r.an + r.bn - 1;
-            J.col0 = cols;
-            cols += J.n_res;
-            J.fa = (u64*)(uintptr_t)(ws * 8);  // offset, rebased below
-            ws += 3 * J.n_res;
-        } else {
-            u64 L = 1ull << it.s;
-            J.sqr = sq;
-            J.n_res = std::min<u64>(r.an + r.bn - 1, L);
-            J.fa = (u64*)(uintptr_t)(ws * 8);
-            for (int i = 0; i < 3; i++) J.r[i] = (const u64*)(uintptr_t)((ws + (u64)i * L) * 8);
-            ws += 3 * L;
-            if (B->buckets.empty() || B->buckets.back().s != it.s) {
-                B->buckets.push_back({it.s, (u32)B->jobs.size(), 0, 0, 0});
-                owner_of.clear();
-            }
-            B->buckets.back().count++;
-            auto key = std::make_pair(r.b, r.bn);
-            auto own = sq ? owner_of.end() : owner_of.find(key);
-            if (own != owner_of.end()) {
-                BJob& O = B->jobs[own->second];
-                O.share = 1;
-                J.share = 2;
-                J.fb = O.fb;
-            } else {
-                J.fb = (u64*)(uintptr_t)(ws * 8);
-                ws += 3 * L;
-                if (!sq) owner_of[key] = (u32)B->jobs.size();
-            }
-        }
-        if (it.s == 0) B->n_school++;
-        B->jobs.push_back(J);
-    }
-    // the batched kernels launch one work-group per (job, tile) item of a bucket
-    for (const Bucket& bk : B->buckets)
-        if ((u64)bk.count * ((1ull << bk.s) >> MIN_MID_BITS) >= (1ull << 31)) {
-            delete B;
-            return fail(NNH_ERR_ARG, "batch bucket too large");
-        }
-    // forward-only entries: transform each shared B once into its owner's fb
-    B->n_req_jobs = (u32)B->jobs.size();
-    for (Bucket& bk : B->buckets) {
-        bk.own_first = (u32)B->jobs.size();
-        for (u32 j = bk.first; j < bk.first + bk.count; j++) {
-            const BJob& O = B->jobs[j];
-            if (O.share != 1) continue;
-            BJob F{};
-            F.a = O.b;
-            F.an = O.bn;
-            F.b = O.b;
-            F.bn = O.bn;
-            F.fa = O.fb;
-            F.fb = O.fb;
-            F.n_res = O.n_res;
-            B->jobs.push_back(F);
-        }
-        bk.own_count = (u32)B->jobs.size() - bk.own_first;
-    }
-    B->ncols = cols;
-    B->ntiles = tiles;
-    B->ws_words = ws;
     int rc;
-    if (B->max_s && (rc = ensure_twiddles(D, B->max_s))) {
+    if (P.max_s && (rc = ensure_twiddles(D, P.max_s))) {
         batch_free(B);
         return rc;
     }
@@ -1078,10 +992,15 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
         B->twf[i] = D.twf[i];
         B->twi[i] = D.twi[i];
     }
+    const u64 ws = P.ws_words, tiles = P.ntiles;
     if (flags & 2) {
         // internal batch: carve the buffers out of the device's reusable workspace
+        if (!P.adds.empty()) {
+            batch_free(B);
+            return fail(NNH_ERR_ARG, "internal batches take no chunked requests");
+        }
         B->borrowed = true;
-        u64 jw = (B->jobs.size() * sizeof(BJob) + 7) / 8, tw = (2 * tiles + 7) / 8 + (B->n_school * 4 + 4 + 7) / 8 + 2;
+        u64 jw = (P.jobs.size() * sizeof(BJob) + 7) / 8, tw = (2 * tiles + 7) / 8 + (P.n_school * 4 + 4 + 7) / 8 + 2;
         if ((rc = ensure(D.bws, ws + 1)) || (rc = ensure(D.bjobs, jw + 1)) || (rc = ensure(D.btiles, tw))) {
             batch_free(B);
             return rc;
@@ -1096,34 +1015,28 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
             if (e != hipSuccess) return fail(NNH_ERR_NOMEM, std::string("batch allocation: ") + hipGetErrorString(e));
             return NNH_OK;
         };
-        if ((rc = alloc((void**)&B->ws, ws * 8)) || (rc = alloc((void**)&B->d_jobs, B->jobs.size() * sizeof(BJob))) ||
-            (rc = alloc((void**)&B->d_tiles, 2 * tiles)) || (rc = alloc((void**)&B->d_sj, B->n_school * 4 + 4))) {
+        if ((rc = alloc((void**)&B->ws, ws * 8)) || (rc = alloc((void**)&B->d_jobs, P.jobs.size() * sizeof(BJob))) ||
+            (rc = alloc((void**)&B->d_tiles, 2 * tiles)) || (rc = alloc((void**)&B->d_sj, P.n_school * 4 + 4)) ||
+            (!P.adds.empty() && ((rc = alloc((void**)&B->d_adds, P.adds.size() * sizeof(AddJob))) ||
+                                 (rc = alloc((void**)&B->d_add_tiles, 2 * P.add_tiles))))) {
             batch_free(B);
             return rc;
         }
     }
-    for (u32 j = 0; j < B->jobs.size(); j++) {
-        BJob& J = B->jobs[j];
-        J.fa = B->ws + (uintptr_t)J.fa / 8;
-        if (J.school) {
-            for (int i = 0; i < 3; i++) J.r[i] = J.fa + (u64)i * J.n_res;
-        } else {
-            J.fb = B->ws + (uintptr_t)J.fb / 8;
-            if (j < B->n_req_jobs)
-                for (int i = 0; i < 3; i++) J.r[i] = B->ws + (uintptr_t)J.r[i] / 8;
-            if (J.sqr) J.fb = J.fa;
-        }
-    }
-    std::vector<u32> sj(B->n_school);
-    for (u32 i = 0; i < B->n_school; i++) sj[i] = i;  // school jobs sort first
-    HIPCHK(hipMemcpy(B->d_jobs, B->jobs.data(), B->jobs.size() * sizeof(BJob), hipMemcpyHostToDevice));
-    if (B->n_school) HIPCHK(hipMemcpy(B->d_sj, sj.data(), sj.size() * 4, hipMemcpyHostToDevice));
+    rebase_plan(P, B->ws);
+    std::vector<u32> sj(P.n_school);
+    for (u32 i = 0; i < P.n_school; i++) sj[i] = i;  // school jobs sort first
+    HIPCHK(hipMemcpy(B->d_jobs, P.jobs.data(), P.jobs.size() * sizeof(BJob), hipMemcpyHostToDevice));
+    if (P.n_school) HIPCHK(hipMemcpy(B->d_sj, sj.data(), sj.size() * 4, hipMemcpyHostToDevice));
+    if (!P.adds.empty())
+        HIPCHK(hipMemcpy(B->d_adds, P.adds.data(), P.adds.size() * sizeof(AddJob), hipMemcpyHostToDevice));
     if (flags & 1) {
         HIPCHK(hipStreamCreateWithFlags(&B->cap_stream, hipStreamNonBlocking));
         HIPCHK(hipStreamBeginCapture(B->cap_stream, hipStreamCaptureModeThreadLocal));
         const int erc = batch_enqueue(D, B, B->cap_stream);
         const hipError_t ce = hipStreamEndCapture(B->cap_stream, &B->graph);
-        const hipError_t ie = (erc || ce != hipSuccess) ? hipSuccess : hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0);
+        const hipError_t ie =
+            (erc || ce != hipSuccess) ? hipSuccess : hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0);
         if (erc || ce != hipSuccess || ie != hipSuccess) {
             batch_free(B);
             if (erc) return erc;
@@ -1135,16 +1048,99 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
 }
 
 // ---------------------------------------------------------------------------
+// The cost-model policy: the network runs on the device once over every
+// point of the policy table (nnh_mlp_forward), and the predictions replace
+// the table the dispatch reads.
+// ---------------------------------------------------------------------------
+static int install_policy(nnh_mlp* m) {
+    const int want = PolicyTable::N_IN * 64 + 64 + 64 * 64 + 64 + N_ARMS * 64 + N_ARMS;
+    if (nnh_mlp_num_params(m) != want) return fail(NNH_ERR_ARG, "policy network must be [3 -> 64 -> 64 -> 6]");
+    std::vector<float> x, y((size_t)PolicyTable::N_POINTS * N_ARMS);
+    PolicyTable::features(x);
+    int rc = nnh_mlp_forward(m, x.data(), PolicyTable::N_POINTS, y.data());
+    if (rc) return fail(rc, "policy network forward pass failed");
+    for (float v : y)
+        if (!std::isfinite(v)) return fail(NNH_ERR_ARG, "policy network predicts a non-finite time");
+    PolicyTable t;
+    t.set(y.data(), N_ARMS);
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    g_policy = std::move(t);
+    return NNH_OK;
+}
+
+// Policy file: a line "nnh-policy 1 <n_in> <n_out> <n_params>", then the
+// network's parameters (nnh_mlp layout), whitespace-separated decimal floats.
+static int load_policy_file(const char* path) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(NNH_ERR_ARG, std::string("cannot open policy file ") + path);
+    char magic[32] = {0};
+    int ver = 0, ni = 0, no = 0, np = 0;
+    int got = std::fscanf(f, "%31s %d %d %d %d", magic, &ver, &ni, &no, &np);
+    std::vector<float> p;
+    if (got == 5 && std::string(magic) == "nnh-policy" && ver == 1 && ni == PolicyTable::N_IN && no == N_ARMS &&
+        np > 0 && np < (1 << 20)) {
+        p.resize(np);
+        for (int k = 0; k < np; k++)
+            if (std::fscanf(f, "%f", &p[k]) != 1) {
+                p.clear();
+                break;
+            }
+    }
+    std::fclose(f);
+    if (p.empty()) return fail(NNH_ERR_ARG, std::string("malformed policy file ") + path);
+    nnh_mlp* m = nullptr;
+    int rc = nnh_mlp_create(ni, no, 1, &m);
+    if (rc) return fail(rc, "policy network allocation failed");
+    rc = nnh_mlp_set_params(m, p.data(), p.size());
+    if (!rc) rc = install_policy(m);
+    else rc = fail(rc, "policy parameters do not fit the network");
+    nnh_mlp_destroy(m);
+    return rc;
+}
+
+// At nnh_init: NNH_POLICY=<file> installs that policy, NNH_POLICY=reference
+// keeps the reference's thresholds, and otherwise the policy shipped beside
+// the library (policy_<arch>.txt) is installed when it exists.
+static void default_policy() {
+    const char* e = std::getenv("NNH_POLICY");
+    std::string path;
+    if (e && *e) {
+        if (std::string(e) == "reference") return;
+        path = e;
+    } else {
+        Dl_info info{};
+        if (!dladdr((void*)&default_policy, &info) || !info.dli_fname) return;
+        std::string lib = info.dli_fname;
+        const size_t cut = lib.find_last_of('/');
+        path = (cut == std::string::npos ? std::string(".") : lib.substr(0, cut)) + "/policy_gfx950.txt";
+        if (FILE* f = std::fopen(path.c_str(), "r")) std::fclose(f);
+        else return;
+    }
+    const std::string keep = g_err;
+    if (load_policy_file(path.c_str()) != NNH_OK)
+        std::fprintf(stderr, "libnnh: %s; using the reference thresholds\n", g_err.c_str());
+    g_err = keep;
+}
+
+// Device set-up, then (once per process) the default policy.
+static std::once_flag g_policy_once;
+static int lib_init(int dev) {
+    int rc = device_init(dev);
+    if (!rc) std::call_once(g_policy_once, default_policy);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
 // Entry points for the device-resident natural layer (nnh_host.hpp).
 // ---------------------------------------------------------------------------
-int host_init() { return g_inited ? NNH_OK : device_init(0); }
+int host_init() { return g_inited ? NNH_OK : lib_init(0); }
 hipStream_t host_stream() { return g_dev.stream; }
 std::mutex& host_mutex() { return g_dev.mu; }
 int host_ws_acquire(hipStream_t st) { return ws_acquire(g_dev, st); }
 void host_ws_release(hipStream_t st) { ws_release(g_dev, st); }
 
 int dev_mul(u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st) {
-    return mul_device(g_dev, out, a, an, b, bn, st);
+    return mul_dispatch(g_dev, out, a, an, b, bn, 1, st);
 }
 int dev_sqr(u64* out, const u64* a, u64 an, hipStream_t st) { return sqr_device(g_dev, out, a, an, st); }
 
@@ -1197,7 +1193,7 @@ using namespace nnh;
 #define NEED_INIT()                                                              \
     do {                                                                         \
         if (!g_inited) {                                                         \
-            int rc_ = device_init(0);                                            \
+            int rc_ = lib_init(0);                                               \
             if (rc_) return rc_;                                                 \
         }                                                                        \
     } while (0)
@@ -1215,9 +1211,55 @@ using namespace nnh;
 
 extern "C" {
 
-int nnh_init(int device) { return device_init(device); }
+int nnh_init(int device) { return lib_init(device); }
 
 int nnh_debug_fail_next(int n) { return g_fail_next.exchange(n > 0 ? n : 0); }
+
+int nnh_policy_set(nnh_mlp* m) {
+    NEED_INIT();
+    if (!m) {
+        std::lock_guard<std::mutex> g(g_dev.mu);
+        g_policy = PolicyTable{};
+        return NNH_OK;
+    }
+    return install_policy(m);
+}
+
+int nnh_policy_load(const char* path) {
+    if (!path) return fail(NNH_ERR_ARG, "null path");
+    NEED_INIT();
+    return load_policy_file(path);
+}
+
+int nnh_policy_active(void) {
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    return g_policy.on ? 1 : 0;
+}
+
+int nnh_policy_arm(uint64_t an, uint64_t bn, uint64_t count, int* arm, int* algo, int* param) {
+    if (!an || !bn || !arm || !algo || !param) return fail(NNH_ERR_ARG, "bad policy query");
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    *arm = g_policy.choose(an, bn, count);
+    if (*arm < 0) {
+        *algo = ALGO_REFERENCE;
+        *param = 0;
+    } else {
+        arm_call(*arm, an, bn, algo, param);
+    }
+    return NNH_OK;
+}
+
+int nnh_arm_feasible(uint64_t an, uint64_t bn, float* feasible) {
+    if (!an || !bn || !feasible) return fail(NNH_ERR_ARG, "bad arguments");
+    arm_feasible(an, bn, feasible);
+    return NNH_OK;
+}
+
+int nnh_arm_call(int arm, uint64_t an, uint64_t bn, int* algo, int* param) {
+    if (arm < 0 || arm >= N_ARMS || !an || !bn || !algo || !param) return fail(NNH_ERR_ARG, "bad arguments");
+    arm_call(arm, an, bn, algo, param);
+    return NNH_OK;
+}
 
 int nnh_shutdown(void) {
     std::lock_guard<std::mutex> g(g_dev.mu);
@@ -1264,7 +1306,7 @@ int nnh_mul_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, cons
                          void* stream) {
     if (!d_dst || !d_a || !d_b || !an || !bn) return fail(NNH_ERR_ARG, "null pointer or empty operand");
     ENTER(stream ? (hipStream_t)stream : g_dev.stream);
-    int rc = mul_device(g_dev, d_dst, d_a, an, d_b, bn, st);
+    int rc = mul_dispatch(g_dev, d_dst, d_a, an, d_b, bn, 1, st);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
     return NNH_OK;
@@ -1307,7 +1349,7 @@ int nnh_mul(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, ui
     u64* dd = db + bn;
     HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
-    if ((rc = mul_device(D, dd, da, an, db, bn, st))) return rc;
+    if ((rc = mul_dispatch(D, dd, da, an, db, bn, 1, st))) return rc;
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(dst, dd, (an + bn) * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1493,10 +1535,12 @@ int nnh_batch_create(const nnh_req* reqs, uint64_t n, int flags, nnh_batch** out
     std::lock_guard<std::mutex> g(g_dev.mu);
     *out = nullptr;
     if (int rc = injected_failure()) return rc;
+    if (flags & ~(NNH_BATCH_GRAPH | NNH_BATCH_REFERENCE)) return fail(NNH_ERR_ARG, "unknown batch flags");
     std::vector<IReq> ir(n);
     for (u64 i = 0; i < n; i++)
         ir[i] = IReq{reqs[i].dst, reqs[i].a, reqs[i].an, reqs[i].b, reqs[i].bn, reqs[i].an + reqs[i].bn, nullptr, 0};
-    return batch_create(g_dev, ir.data(), n, flags, out);
+    return batch_create(g_dev, ir.data(), n, (flags & NNH_BATCH_GRAPH ? 1 : 0) | (flags & NNH_BATCH_REFERENCE ? 0 : 4),
+                        out);
 }
 
 int nnh_batch_run(nnh_batch* b, void* stream) {

@@ -363,6 +363,6 @@ void launch_crt_batch(const BJob* jobs, u32 njobs, u64 ntiles, u8* tile_fn, u8* 
                        (const u8*)tile_cin);
 }
 
-u64 crt_tiles(u64 n_out) { return (n_out + 256 * CRT_TILE_ROUNDS - 1) / (256 * CRT_TILE_ROUNDS); }
+static_assert(256 * CRT_TILE_ROUNDS == CRT_TILE_WORDS, "CRT tile size");
 
 }  // namespace nnh

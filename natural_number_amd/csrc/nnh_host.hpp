@@ -35,18 +35,6 @@ int dev_sqr(u64* out, const u64* a, u64 an, hipStream_t st);
 int dev_mul_scale(u64* out, const u64* a, u64 an, const u64* b, u64 bn, int s, u64 save_size, u64* n_out,
                   hipStream_t st);
 
-// One request of an internal batch: dst[0 .. n_out) = a * b + add (the
-// value must fit n_out words; n_out <= an + bn).
-struct IReq {
-    u64* dst;
-    const u64* a;
-    u64 an;
-    const u64* b;
-    u64 bn;
-    u64 n_out;
-    const u64* add;
-    u64 n_add;
-};
 // Plan, enqueue, synchronise and release one batch.
 int dev_batch(const IReq* reqs, u64 n, hipStream_t st);
 

@@ -3,11 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
-namespace nnh {
+#include "nnh_jobs.hpp"
 
-typedef uint64_t u64;
-typedef uint32_t u32;
-typedef uint8_t u8;
+namespace nnh {
 
 // Root bases in Montgomery form (x * 2^64 mod p), by value in kernel args.
 struct RootBases {
@@ -65,26 +63,6 @@ struct CrtDesc {
     CrtConsts C;
 };
 
-// One product of a batch (device-resident table).
-struct BJob {
-    const u64* a;
-    const u64* b;
-    u64 an, bn;
-    u64* fa;          // transform workspace 3L (or school columns 3 n_res)
-    u64* fb;          // second transform 3L
-    const u64* r[3];  // CRT inputs: fa + i L, or the three column arrays
-    u64 n_res;        // positions with residues / columns
-    u64* out;
-    u64 n_out;
-    u64 tile0;        // first global CRT tile of this job
-    u64 col0;         // first global school column (school jobs)
-    const u64* add;   // optional addend (n_add words) summed into out
-    u64 n_add;
-    u32 school;       // 1: schoolbook columns
-    u32 sqr;          // 1: b == a (single transform)
-    u32 share;        // B transform shared within the batch: 0 no, 1 owner, 2 reader of the owner's fb
-};
-
 struct TopBatchParams {
     const u64* tw[3];
     u64 alpha_m[3];
@@ -114,7 +92,6 @@ void launch_twiddles(u64* tw, u64 n, const RootBases& rb, int pr, int inv, hipSt
 void launch_top_pass(const TopPassDesc& d, hipStream_t st, hipStream_t st2 = nullptr);
 void launch_mid(const MidDesc& d, hipStream_t st, hipStream_t st2 = nullptr);
 void launch_crt(const CrtDesc& d, hipStream_t st);
-u64 crt_tiles(u64 n_out);
 void launch_school_cols(const u64* a, u64 an, const u64* b, u64 bn, u64* c0, u64* c1, u64* c2, hipStream_t st);
 void launch_cyclic_fold(u64* w, u64 L, hipStream_t st);
 // One Karatsuba level over schoolbook columns (an >= bn >= 2): writes the

@@ -5,24 +5,6 @@
 
 namespace nnh {
 
-// One modular addition of a batch: out[0..n) = a + b (+ carry-in) or
-// a - b (a + ~b + 1), with a zero-extended beyond na and b beyond nb; the
-// carry out of word n-1 is dropped (stored to *cout when non-null: for a
-// subtraction 1 means no borrow).  out may alias a or b.
-struct AddJob {
-    u64* out;
-    const u64* a;
-    const u64* b;
-    u64 n, na, nb;
-    u64 tile0;            // first 4096-word tile of this job (filled by the launcher)
-    const u32* cin_ptr;   // device carry-in (add only), overrides cin when non-null
-    u32* cout;
-    u32 sub;
-    u32 cin;
-};
-
-constexpr u64 LIMB_TILE = 4096;
-
 // Enqueue a batch of additions; jobs/tile scratch are device buffers of at
 // least njobs AddJobs and 2*ntiles bytes (ntiles = sum ceil(n/4096)).
 u64 add_tiles_needed(const AddJob* jobs, u64 njobs);
