@@ -78,30 +78,44 @@ def test_batch_mixed_vs_oracle(nnh, oracle):
         b.close()
 
 
-def test_batch_config3_matches_single_path(nnh):
-    """All 4096 products of the uneven mix: batched (graph) == one call per product,
-    and every product checked modulo two primes independently of the GPU."""
+def test_batch_config3_vs_oracle(nnh, oracle):
+    """All 4096 products of the uneven mix, batched under a hipGraph with the
+    installed policy's arms and with the reference's rule, and through one
+    call per product: every product of each compared word for word with the
+    oracle (16 host threads; the oracle's twiddle tables grown first)."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+
     from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
 
     sizes = batch_mix_sizes()
     ops = list(batch_mix_operands(sizes))
-    say(f"config3 batch: {len(ops)} products, {sum(a + b for a, b in sizes)} words")
+    say(f"config3 batch: {len(ops)} products, {sum(a + b for a, b in sizes)} words, policy {nnh.policy_active()}")
     db = DevBatch(nnh, ops)
-    b = nnh.Batch(db.reqs, graph=True)
-    b.run()
-    nnh.synchronize()
-    batched = [r.copy() for r in db.results()]
+    results = {}
+    for name, ref in (("policy", False), ("reference rule", True)):
+        db.clear()
+        b = nnh.Batch(db.reqs, graph=True, reference=ref)
+        b.run()
+        nnh.synchronize()
+        results[name] = [r.copy() for r in db.results()]
+        b.close()
     db.clear()
     for r in db.reqs:
         nnh.mul_device_async(*r, None)
     nnh.synchronize()
-    single = db.results()
-    for i, (x, y) in enumerate(zip(batched, single)):
-        assert np.array_equal(x, y), i
-    for q in (2**61 - 1, (1 << 64) - 59):
-        for (a, bb), got in zip(ops[::7], batched[::7]):
-            assert to_int(got) % q == (to_int(a) % q) * (to_int(bb) % q) % q
-    b.close()
+    results["single"] = db.results()
+    smax = max((an + bn - 2).bit_length() for an, bn in sizes)
+    tw = np.empty(1 << (smax - 1), dtype=np.uint64)
+    oracle.L.nno_twiddles(tw.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 0, smax, 0)
+
+    def check(i):
+        exp = oracle.mul(*ops[i])
+        return [k for k, v in results.items() if not np.array_equal(v[i], exp)]
+
+    with ThreadPoolExecutor(16) as ex:
+        bad = [(i, b) for i, b in enumerate(ex.map(check, range(len(ops)))) if b]
+    assert not bad, bad[:10]
 
 
 def test_batch_shared_b_transform(nnh, oracle):

@@ -3,7 +3,7 @@ timings over a 10 x 10 grid of operand sizes up to 2^22 bits, the FP32 MLP
 trained on the GPU by the library's kernels, and 300 held-out log-uniform
 requests timed exhaustively (every feasible arm: the judge).  The trained
 network must lower the regret of the untrained one, and its picks must not
-cost more than the reference's fixed-threshold dispatch (NNH_ALGO_AUTO, the
+cost more than the reference's fixed-threshold dispatch (NNH_ALGO_REFERENCE, the
 method without a model) beyond timing noise; the gain is printed.  Every arm
 is exact, so only time is at stake (the arms' exactness is tested against
 the oracle in test_gpu_algos.py)."""
@@ -14,6 +14,17 @@ pytestmark = pytest.mark.gpu
 
 def say(*a):
     print(*a, flush=True)
+
+
+def _restore_shipped_policy(nnh):
+    import os
+
+    shipped = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "natural_number_amd",
+                           "policy_gfx950.txt")
+    if os.path.exists(shipped):
+        nnh.policy_load(shipped)
+    else:
+        nnh.policy_clear()
 
 
 def test_trained_model_gain_over_reference_policy(nnh):
@@ -37,8 +48,56 @@ def test_trained_model_gain_over_reference_policy(nnh):
         model.close()
     finally:
         bench.close()
+        _restore_shipped_policy(nnh)
     gain = ev["total_ms_reference_policy"] / ev["total_ms_model"]
     say(f"train loss {loss:.4f}; regret trained {ev['regret_total']:.4f}, untrained {ev0['regret_total']:.4f}, "
         f"reference policy {ev['reference_policy_regret_total']:.4f}; gain over the reference policy {gain:.4f}")
     assert ev["regret_total"] < ev0["regret_total"]
     assert ev["total_ms_model"] <= 1.05 * ev["total_ms_reference_policy"]
+
+
+def test_config5_at_its_size(nnh, oracle):
+    """Config 5 as stated: timings over the 1010-point log-uniform grid of
+    2^10 .. 2^30-bit operands at batch counts 1/8/64, the MLP trained on the
+    GPU by the library's kernels, installed as the library's dispatch
+    policy, and 10^4 held-out log-uniform requests (seed 0x4E4E2E1972800005)
+    timed exhaustively: the policy's regret against the best arm is at most
+    5 % of the total time and it beats the reference's fixed thresholds.
+    Products the installed policy dispatches are bit-exact (a sample of the
+    held-out shapes against the oracle)."""
+    import numpy as np
+
+    from conftest import rand_limbs
+    from natural_number_amd import costmodel as cm
+    from natural_number_amd.workloads import log_uniform_requests
+
+    bench = cm.Bench()
+    model = cm.CostModel(seed=7)
+    try:
+        pts = cm.grid(28)
+        say(f"config 5: timing {len(pts)} grid points")
+        X, T, M = cm.collect(bench, pts)
+        for lr in (3e-3, 1e-3, 3e-4):
+            loss = model.train(X, T, M, steps=4000, lr=lr)
+        reqs = log_uniform_requests(10000, 0x4E4E2E1972800005)
+        say("config 5: evaluating 10^4 held-out requests")
+        ev = cm.evaluate(bench, model, reqs)
+        say(f"loss {loss:.4f} regret {ev['regret_total']:.4f} (exact inference {ev['regret_total_exact_inference']:.4f})"
+            f" reference policy {ev['reference_policy_regret_total']:.4f} gain {ev['gain_over_reference_policy']:.4f}"
+            f" best picked {ev['picked_best_fraction']:.3f}")
+        assert len(pts) == 1010
+        assert ev["regret_total"] <= 0.05
+        assert ev["gain_over_reference_policy"] > 1.0
+        # the trained policy is installed: its products against the oracle
+        rng = np.random.default_rng(5)
+        small = [(a, b) for a, b in reqs if max(a, b) <= 1 << 17][:150]
+        arms = set()
+        for an, bn in small:
+            arms.add(nnh.policy_arm(an, bn)[0])
+            x, y = rand_limbs(rng, an), rand_limbs(rng, bn)
+            assert np.array_equal(nnh.mul(x, y), oracle.mul(x, y)), (an, bn)
+        say(f"arms taken on the sample: {sorted(arms)}")
+    finally:
+        bench.close()
+        model.close()
+        _restore_shipped_policy(nnh)

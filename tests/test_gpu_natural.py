@@ -167,7 +167,8 @@ def test_calc_e_small_exact(nnh):
     assert to_int(den) == D and to_int(num) == N
 
 
-@pytest.mark.parametrize("name", ["e_k14_D10000", "e_k17_D100000", "e_k18_D1000000", "e_k21_D10000000"])
+@pytest.mark.parametrize("name", ["e_k14_D10000", "e_k17_D100000", "e_k18_D1000000", "e_k21_D10000000",
+                                  "e_k24_D100000000"])
 def test_e_digits_golden(nnh, name):
     g = GOLDEN[name]
     s, ms = nnh.e_digits(g["k"], g["digits"])
