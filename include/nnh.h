@@ -87,6 +87,15 @@ int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an,
  * warm-up, on the library stream (device pointers). */
 int nnh_time_algo(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint64_t* d_b, uint64_t bn, int algo,
                   int param, int count, int reps, float* ms_per_product);
+/* Autotuner timing in batch context: device milliseconds per product of a
+ * batch of `count` an x bn products (operands at different offsets of the
+ * a_cap / b_cap-word buffers, outputs consecutive in the dst_cap-word
+ * buffer, count * (an + bn) <= dst_cap) planned with every request on cost-
+ * model arm `arm` (0..5, see nnh_policy_set) and replayed as a hipGraph
+ * `reps` times after one warm-up. */
+int nnh_time_batch(uint64_t* d_dst, uint64_t dst_cap, const uint64_t* d_a, uint64_t a_cap, uint64_t an,
+                   const uint64_t* d_b, uint64_t b_cap, uint64_t bn, int arm, int count, int reps,
+                   float* ms_per_product);
 
 /* Learned cost model (no counterpart in the reference): an FP32 MLP
  * [n_in -> 64 -> 64 -> n_out] with ReLU, trained on the device by

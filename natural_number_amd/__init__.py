@@ -68,6 +68,7 @@ def lib():
         "nnh_mul_algo": ([_u64p, _u64p, u64, _u64p, u64, i32, i32], i32),
         "nnh_mul_algo_device_async": ([vp, vp, u64, vp, u64, i32, i32, vp], i32),
         "nnh_time_algo": ([vp, vp, u64, vp, u64, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_float)], i32),
+        "nnh_time_batch": ([vp, u64, vp, u64, u64, vp, u64, u64, i32, i32, i32, ctypes.POINTER(ctypes.c_float)], i32),
         "nnh_butterfly_peak": ([i32, u64, ctypes.POINTER(ctypes.c_double)], i32),
         "nnh_transform_plan": ([i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)], i32),
         "nnh_mlp_create": ([i32, i32, u64, ctypes.POINTER(vp)], i32),

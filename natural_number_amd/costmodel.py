@@ -80,14 +80,29 @@ def features(an, bn, count=1):
 
 class Bench:
     """Random operands of the largest size, resident on the device; timed
-    products point into them."""
+    products point into them.  Batch timings place the batch's requests at
+    different offsets (64 spare words, so even the largest operands of a
+    batch have distinct B pointers) and their outputs side by side."""
 
-    def __init__(self, seed=0x4E4E2E19, max_words=MAX_WORDS):
+    def __init__(self, seed=0x4E4E2E19, max_words=MAX_WORDS, out_words=None):
         rng = np.random.default_rng(seed)
         self.max = max_words
-        self.a = DeviceWords(max_words, rng.integers(0, 2**64, size=max_words, dtype=np.uint64))
-        self.b = DeviceWords(max_words, rng.integers(0, 2**64, size=max_words, dtype=np.uint64))
-        self.out = DeviceWords(2 * max_words)
+        self.cap = max_words + 64
+        self.a = DeviceWords(self.cap, rng.integers(0, 2**64, size=self.cap, dtype=np.uint64))
+        self.b = DeviceWords(self.cap, rng.integers(0, 2**64, size=self.cap, dtype=np.uint64))
+        self.out_words = out_words or 8 * max_words
+        self.out = DeviceWords(self.out_words)
+
+    def time_batch(self, arm, an, bn, count, reps=None):
+        """ms per product of a batch of `count` an x bn products, every request
+        on `arm` (nnh_time_batch: planned once, replayed as a graph)."""
+        work = count * (an + bn)
+        if reps is None:
+            reps = 8 if work < (1 << 20) else (3 if work < (1 << 24) else 1)
+        ms = ctypes.c_float()
+        _check(lib().nnh_time_batch(self.out.ptr, self.out_words, self.a.ptr, self.cap, an, self.b.ptr, self.cap, bn,
+                                    arm, count, reps, ctypes.byref(ms)))
+        return float(ms.value)
 
     def time(self, algo, param, an, bn, count=1, reps=None):
         if reps is None:
@@ -107,7 +122,8 @@ class Bench:
 
 
 def grid(n_side=20, lo_bits=10, hi_bits=30, counts=(1, 8, 64)):
-    """Training grid: log-uniform (larger, smaller) operand sizes and batch counts."""
+    """Training grid: log-uniform (larger, smaller) operand sizes and batch
+    counts (1: a product on its own; 8, 64: that many products in one batch)."""
     pts = []
     for i in range(n_side):
         for j in range(i + 1):
@@ -117,6 +133,15 @@ def grid(n_side=20, lo_bits=10, hi_bits=30, counts=(1, 8, 64)):
                 if c * big <= (1 << 26):
                     pts.append((big, small, c))
     return pts
+
+
+def time_point(bench, arm, an, bn, count):
+    """ms per product of arm at this grid point: on its own (count 1,
+    nnh_time_algo) or inside a batch of `count` (nnh_time_batch)."""
+    if count == 1:
+        algo, param = arm_call(arm, an, bn)
+        return bench.time(algo, param, an, bn)
+    return bench.time_batch(arm, an, bn, count)
 
 
 def collect(bench, points, log=None, repeats=2):
@@ -131,8 +156,7 @@ def collect(bench, points, log=None, repeats=2):
         f = feasible(an, bn)
         for arm in range(N_ARMS):
             if f[arm]:
-                algo, param = arm_call(arm, an, bn)
-                ms = min(bench.time(algo, param, an, bn, count=c) for _ in range(repeats))
+                ms = min(time_point(bench, arm, an, bn, c) for _ in range(repeats))
                 T[k, arm] = math.log2(max(ms, 1e-6))
                 M[k, arm] = 1.0
         if log and k % 50 == 0:

@@ -972,13 +972,14 @@ static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
 
 // flags bit 0: capture into a hipGraph; bit 1: internal batch (buffers from
 // the device's reusable batch workspace; no policy); bit 2: plan each
-// request's arm with the installed policy.
-static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch** out) {
+// request's arm with the installed policy.  forced_arm >= 0: every request
+// runs that arm (timing).
+static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch** out, int forced_arm = -1) {
     nnh_batch* B = new nnh_batch();
     BatchPlan& P = B->plan;
     std::string err;
     const bool use_policy = (flags & 4) && !(flags & 2) && g_policy.on;
-    if (plan_batch(reqs, n, use_policy ? &g_policy : nullptr, std::min<u64>(n, 64), P, err)) {
+    if (plan_batch(reqs, n, use_policy ? &g_policy : nullptr, std::min<u64>(n, 64), P, err, forced_arm)) {
         delete B;
         return fail(NNH_ERR_ARG, err);
     }
@@ -994,21 +995,22 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
     }
     const u64 ws = P.ws_words, tiles = P.ntiles;
     if (flags & 2) {
-        // internal batch: carve the buffers out of the device's reusable workspace
-        if (!P.adds.empty()) {
-            batch_free(B);
-            return fail(NNH_ERR_ARG, "internal batches take no chunked requests");
-        }
+        // internal batch: carve the buffers out of the device's reusable
+        // workspace (job table, then the additions; CRT tile flags, then the
+        // schoolbook list, then the additions' tile flags)
         B->borrowed = true;
-        u64 jw = (P.jobs.size() * sizeof(BJob) + 7) / 8, tw = (2 * tiles + 7) / 8 + (P.n_school * 4 + 4 + 7) / 8 + 2;
-        if ((rc = ensure(D.bws, ws + 1)) || (rc = ensure(D.bjobs, jw + 1)) || (rc = ensure(D.btiles, tw))) {
+        const u64 jw = (P.jobs.size() * sizeof(BJob) + 7) / 8, aw = (P.adds.size() * sizeof(AddJob) + 7) / 8;
+        const u64 tw = (2 * tiles + 7) / 8 + 1, sw = (P.n_school * 4 + 4 + 7) / 8 + 1, atw = (2 * P.add_tiles + 7) / 8;
+        if ((rc = ensure(D.bws, ws + 1)) || (rc = ensure(D.bjobs, jw + aw + 1)) || (rc = ensure(D.btiles, tw + sw + atw + 1))) {
             batch_free(B);
             return rc;
         }
         B->ws = D.bws.p;
         B->d_jobs = (BJob*)D.bjobs.p;
+        B->d_adds = (AddJob*)(D.bjobs.p + jw);
         B->d_tiles = (u8*)D.btiles.p;
-        B->d_sj = (u32*)(D.btiles.p + (2 * tiles + 7) / 8 + 1);
+        B->d_sj = (u32*)(D.btiles.p + tw);
+        B->d_add_tiles = (u8*)(D.btiles.p + tw + sw);
     } else {
         auto alloc = [&](void** p, u64 bytes) -> int {
             hipError_t e = hipMalloc(p, std::max<u64>(bytes, 64));
@@ -1411,6 +1413,47 @@ int nnh_time_algo(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, const uint6
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *ms_per_product = ms / (float)(reps * count);
+    return NNH_OK;
+}
+
+int nnh_time_batch(uint64_t* d_dst, uint64_t dst_cap, const uint64_t* d_a, uint64_t a_cap, uint64_t an,
+                   const uint64_t* d_b, uint64_t b_cap, uint64_t bn, int arm, int count, int reps,
+                   float* ms_per_product) {
+    if (!d_dst || !d_a || !d_b || !an || !bn || an > a_cap || bn > b_cap || arm < 0 || arm >= N_ARMS || count < 1 ||
+        reps < 1 || !ms_per_product || (u64)count * (an + bn) > dst_cap)
+        return fail(NNH_ERR_ARG, "bad batch timing arguments");
+    ENTER(g_dev.stream);
+    Device& D = g_dev;
+    // count requests with operands at different offsets (distinct B
+    // pointers, so that no transform is shared) and disjoint outputs
+    std::vector<IReq> rq(count);
+    for (int i = 0; i < count; i++) {
+        const u64 oa = ((u64)i * an) % (a_cap - an + 1), ob = ((u64)i * bn + i) % (b_cap - bn + 1);
+        rq[i] = IReq{d_dst + (u64)i * (an + bn), d_a + oa, an, d_b + ob, bn, an + bn, nullptr, 0};
+    }
+    nnh_batch* B = nullptr;
+    int rc = batch_create(D, rq.data(), rq.size(), 1 | 2, &B, arm);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0;
+    auto run = [&]() -> int {
+        HIPCHK(hipGraphLaunch(B->exec, st));  // warm-up
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(B->exec, st));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        return NNH_OK;
+    };
+    rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    batch_free(B);
     if (rc) return rc;
     *ms_per_product = ms / (float)(reps * count);
     return NNH_OK;

@@ -126,7 +126,8 @@ u64 wsoff(const void* p) { return static_cast<u64>(reinterpret_cast<uintptr_t>(p
 
 }  // namespace
 
-int plan_batch(const IReq* reqs, u64 n, const PolicyTable* pol, u64 count, BatchPlan& P, std::string& err) {
+int plan_batch(const IReq* reqs, u64 n, const PolicyTable* pol, u64 count, BatchPlan& P, std::string& err,
+               int forced_arm) {
     P = BatchPlan{};
     std::vector<Route> route(n);
     P.arms.assign(n, -1);
@@ -142,7 +143,9 @@ int plan_batch(const IReq* reqs, u64 n, const PolicyTable* pol, u64 count, Batch
         Route& R = route[i];
         const u64 mn = std::min(r.an, r.bn);
         int arm = -1;
-        if (pol && pol->on && !sq && !r.n_add && r.n_out == r.an + r.bn) arm = pol->choose(r.an, r.bn, count);
+        const bool eligible = !sq && !r.n_add && r.n_out == r.an + r.bn;
+        if (eligible && forced_arm >= 0 && forced_arm < N_ARMS) arm = forced_arm;
+        else if (eligible && pol && pol->on) arm = pol->choose(r.an, r.bn, count);
         P.arms[i] = arm;
         if (arm < 0) {
             R.s = mn < NTT_MUL_THRESHOLD ? 0 : ntt_scale(r.an, r.bn);

@@ -54,7 +54,8 @@ void arm_call(int arm, u64 an, u64 bn, int* algo, int* param);
 // The cost model compiled into a table: predicted log2 milliseconds of
 // every arm at grid points (larger, smaller operand words) spaced 1/Q
 // octave over [2^LO, 2^HI] words (the model's training range, 2^10 .. 2^30
-// bits) and at the counts of back-to-back products it was trained on.  The
+// bits) and at the counts it was trained on: 1 (a product on its own) and
+// 8, 64 (products inside one batch of that many).  The
 // network itself runs on the device when the table is built (one forward
 // launch over every point); a request is then a table lookup and an
 // arg-min over the arms feasible at its exact sizes.
@@ -76,7 +77,7 @@ struct PolicyTable {
     // over 30, log2 count over 8).
     static void request_features(u64 an, u64 bn, u64 count, float x[N_IN]);
     void set(const float* y, u64 n_out_per_point);
-    // Arm for an an x bn product among `count` back-to-back ones; -1 when no
+    // Arm for an an x bn product alone (count 1) or in a batch of `count`; -1 when no
     // table is installed or the sizes lie outside it (the reference's
     // thresholds then apply).
     int choose(u64 an, u64 bn, u64 count) const;
@@ -111,12 +112,15 @@ struct BatchPlan {
 };
 
 // Plan n requests.  With a policy (non-null, installed), requests without
-// an addend whose result is an + bn words run the arm it picks at `count`
-// back-to-back products: schoolbook (also for Karatsuba), one transform, or
+// an addend whose result is an + bn words run the arm it picks for a batch
+// of `count`: schoolbook (also for Karatsuba), one transform, or
 // the chunked transform (the reference split is the chunked form at the
 // split's scale).  Otherwise schoolbook below 384 words and one transform
 // of get_NTT_scale words above.  Returns 0 or an nnh_status with `err`.
-int plan_batch(const IReq* reqs, u64 n, const PolicyTable* pol, u64 count, BatchPlan& P, std::string& err);
+// forced_arm >= 0 runs every eligible request with that arm instead (timing
+// of the arms in batch context).
+int plan_batch(const IReq* reqs, u64 n, const PolicyTable* pol, u64 count, BatchPlan& P, std::string& err,
+               int forced_arm = -1);
 // Replace workspace offsets by addresses in ws.
 void rebase_plan(BatchPlan& P, u64* ws);
 
