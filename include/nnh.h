@@ -12,8 +12,9 @@
  * The entry points replace the reference's multiply choice points:
  *   natural::_mul / _mul_NTT     natural.h:391-450   -> nnh_mul
  *   natural::_sqr / _sqr_NTT     natural.h:433-442, 529-535 -> nnh_sqr
- *   NTT_info::load+NTT           ntt_info.h:147-198  -> nnh_transform_forward
- *   mul_load_NTT_info (reuse)    natural.h:504-527   -> nnh_mul_transformed
+ *   NTT_info::load+NTT           ntt_info.h:147-198  -> nnh_xf_forward (device), nnh_transform_forward (host)
+ *   mul_save/mul_load/sqr_save   natural.h:452-527   -> nnh_xf_mul, nnh_xf_mul2, nnh_xf_sqr (device),
+ *                                                       nnh_mul_transformed (host)
  *   NTT_info::save (cyclic mode) ntt_info.h:237-248  -> nnh_mul_cyclic
  * and, with no counterpart in the reference (which multiplies one pair at
  * a time), nnh_mul_batch for many independent products.
@@ -190,6 +191,31 @@ int nnh_alpha(int scale, uint64_t out[3]);
  * nnh_transform_forward (mul_load_NTT_info, natural.h:504-515).  Requires
  * an + bn - 1 <= 2^scale.  Host buffers. */
 int nnh_mul_transformed(uint64_t* dst, const uint64_t* fa, uint64_t an, const uint64_t* b, uint64_t bn, int scale);
+
+/* Transform handles: a forward transform kept in device memory, the
+ * counterpart of the NTT_info that the reference's mul_save_NTT_info /
+ * sqr_save_NTT_info keep and mul_load_NTT_info reuses (natural.h:452-527).
+ * A handle holds 3 * 2^scale words (8 <= scale <= 30); the caller keeps it
+ * as long as it likes (nnh_shutdown does not free it).  Device pointers,
+ * work queued on `stream` (NULL = the library's stream).
+ *   nnh_xf_forward  NTT_info(a, scale) + NTT(): keep a's transform (an <= 2^scale)
+ *   nnh_xf_mul      d = a * b with a's kept transform (mul_load_NTT_info; with
+ *                   nnh_xf_forward first, mul_save_NTT_info), bn <= 2^scale
+ *   nnh_xf_mul2     d = a * b from two kept transforms of one scale
+ *                   (mul_load_NTT_info with ni_a and ni_b)
+ *   nnh_xf_sqr      d = a^2 from a's kept transform (sqr_save_NTT_info)
+ * Every product follows NTT_info::save (ntt_info.h:237-248): with
+ * size = an + bn words, when size - 1 > 2^scale the result is the product
+ * modulo B^(2^scale) - 1 in 2^scale words, otherwise the exact product in
+ * size words; *n_out receives that count.  d_dst holds max(size, 2^scale + 2)
+ * words and must not overlap an operand.  The result is not normalised. */
+typedef struct nnh_xf nnh_xf;
+int nnh_xf_create(int scale, nnh_xf** out);
+int nnh_xf_destroy(nnh_xf* xf);
+int nnh_xf_forward(nnh_xf* xf, const uint64_t* d_a, uint64_t an, void* stream);
+int nnh_xf_mul(uint64_t* d_dst, uint64_t* n_out, const nnh_xf* xa, const uint64_t* d_b, uint64_t bn, void* stream);
+int nnh_xf_mul2(uint64_t* d_dst, uint64_t* n_out, const nnh_xf* xa, const nnh_xf* xb, void* stream);
+int nnh_xf_sqr(uint64_t* d_dst, uint64_t* n_out, const nnh_xf* xa, void* stream);
 
 /* Batched products (no counterpart in the reference, which multiplies one
  * pair at a time; needed for binary-splitting workloads such as calc_e,

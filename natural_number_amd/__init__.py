@@ -80,6 +80,12 @@ def lib():
         "nnh_mlp_gradients": ([vp, vp, vp, vp, u64, vp, ctypes.POINTER(ctypes.c_float)], i32),
         "nnh_mlp_train": ([vp, vp, vp, vp, u64, i32, ctypes.c_float, ctypes.POINTER(ctypes.c_float)], i32),
         "nnh_mlp_choose": ([vp, vp, vp, u64, vp], i32),
+        "nnh_xf_create": ([i32, ctypes.POINTER(vp)], i32),
+        "nnh_xf_destroy": ([vp], i32),
+        "nnh_xf_forward": ([vp, vp, u64, vp], i32),
+        "nnh_xf_mul": ([vp, _u64p, vp, vp, u64, vp], i32),
+        "nnh_xf_mul2": ([vp, _u64p, vp, vp, vp], i32),
+        "nnh_xf_sqr": ([vp, _u64p, vp, vp], i32),
         "nnh_policy_set": ([vp], i32),
         "nnh_policy_load": ([ctypes.c_char_p], i32),
         "nnh_policy_active": ([], i32),
@@ -255,6 +261,48 @@ class Batch:
     def close(self):
         if self.h:
             _check(lib().nnh_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Transform:
+    """A forward transform kept in device memory (nnh_xf_*): the library's
+    counterpart of a kept NTT_info (mul_save / mul_load / sqr_save,
+    natural.h:452-527)."""
+
+    def __init__(self, scale):
+        h = ctypes.c_void_p()
+        _check(lib().nnh_xf_create(scale, ctypes.byref(h)))
+        self.h = h
+        self.scale = scale
+
+    def forward(self, d_a, an, stream=None):
+        _check(lib().nnh_xf_forward(self.h, d_a, an, stream))
+
+    def mul(self, d_dst, d_b, bn, stream=None):
+        """d_dst = a * b (save rule); returns the result's word count."""
+        n = ctypes.c_uint64()
+        _check(lib().nnh_xf_mul(d_dst, ctypes.byref(n), self.h, d_b, bn, stream))
+        return n.value
+
+    def mul2(self, d_dst, other, stream=None):
+        n = ctypes.c_uint64()
+        _check(lib().nnh_xf_mul2(d_dst, ctypes.byref(n), self.h, other.h, stream))
+        return n.value
+
+    def sqr(self, d_dst, stream=None):
+        n = ctypes.c_uint64()
+        _check(lib().nnh_xf_sqr(d_dst, ctypes.byref(n), self.h, stream))
+        return n.value
+
+    def close(self):
+        if self.h:
+            _check(lib().nnh_xf_destroy(self.h))
             self.h = None
 
     def __del__(self):

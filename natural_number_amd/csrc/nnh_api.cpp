@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -620,6 +621,23 @@ static int product_core(Device& D, const Plan& pl, const u64* a, u64 an, const u
     return crt(D, pl, fa, n_res, out, n_out, addend, n_add, st);
 }
 
+// Product of operands whose transforms are complete: A's (3 L words at
+// fa_src) is copied into the workspace, which the product overwrites; B's
+// is read where it is (fb_src), or A's is squared.
+static int product_transformed(Device& D, const Plan& pl, const u64* fa_src, u64 an, const u64* fb_src, u64 bn,
+                               bool sqr, u64* out, u64 n_out, hipStream_t st) {
+    u64* fa = D.ws.p;
+    HIPCHK(hipMemcpyAsync(fa, fa_src, 3 * pl.L * sizeof(u64), hipMemcpyDeviceToDevice, st));
+    hipStream_t st2;
+    int rc = fork_p2(D, st, &st2);
+    if (rc) return rc;
+    mid(D, pl, fa, sqr ? fa : fb_src, nullptr, an, nullptr, sqr ? an : bn, sqr, false, true, !sqr, st, st2);
+    if (pl.ntop) inv_top(D, pl, fa, st, st2);
+    if ((rc = join_p2(D, st, st2))) return rc;
+    const u64 n_res = std::min<u64>(an + (sqr ? an : bn) - 1, pl.L);
+    return crt(D, pl, fa, n_res, out, n_out, nullptr, 0, st);
+}
+
 // Complete forward transform of x into dst (3 L words).
 static int transform_full(Device& D, const Plan& pl, const u64* x, u64 xn, u64* dst, hipStream_t st) {
     bool small = pl.ntop == 0;
@@ -876,6 +894,14 @@ static int stage_host(Device& D, u64 words) { return ensure(D.stage, words); }
 // one workspace; enqueued directly or captured once into a hipGraph.
 // ---------------------------------------------------------------------------
 }  // namespace nnh
+
+// A forward transform kept in device memory (the counterpart of an NTT_info
+// kept by mul_save_NTT_info / sqr_save_NTT_info, natural.h:452-501).
+struct nnh_xf {
+    int scale = 0;
+    nnh::u64 held = 0;       // words of the transformed operand (0: none yet)
+    nnh::u64* f = nullptr;   // 3 * 2^scale words, prime-major
+};
 
 struct nnh_batch {
     nnh::BatchPlan plan;
@@ -1690,6 +1716,102 @@ int nnh_mul_cyclic(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t
     HIPCHK(hipMemcpyAsync(dst, dd, (cyclic ? L : an + bn) * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return NNH_OK;
+}
+
+
+// ---- transform handles (mul_save / mul_load / sqr_save, natural.h:452-527) --
+
+int nnh_xf_create(int scale, nnh_xf** out) {
+    if (!out || scale < MIN_MID_BITS || scale > MAX_SCALE) return fail(NNH_ERR_ARG, "bad transform scale");
+    NEED_INIT();
+    *out = nullptr;
+    nnh_xf* x = new nnh_xf();
+    x->scale = scale;
+    hipError_t e = hipMalloc((void**)&x->f, (3ull << scale) * sizeof(u64));
+    if (e != hipSuccess) {
+        delete x;
+        return fail(e == hipErrorOutOfMemory ? NNH_ERR_NOMEM : NNH_ERR_HIP,
+                    std::string("transform allocation: ") + hipGetErrorString(e));
+    }
+    *out = x;
+    return NNH_OK;
+}
+
+int nnh_xf_destroy(nnh_xf* x) {
+    if (!x) return NNH_OK;
+    (void)hipDeviceSynchronize();  // queued work may still read the transform
+    (void)hipFree(x->f);
+    delete x;
+    return NNH_OK;
+}
+
+int nnh_xf_forward(nnh_xf* x, const uint64_t* d_a, uint64_t an, void* stream) {
+    if (!x || !d_a || !an || an > (1ull << x->scale)) return fail(NNH_ERR_ARG, "bad transform arguments");
+    ENTER(stream ? (hipStream_t)stream : g_dev.stream);
+    Device& D = g_dev;
+    Plan pl = make_plan(D, x->scale);
+    int rc;
+    if ((rc = ensure_twiddles(D, x->scale))) return rc;
+    if ((rc = transform_full(D, pl, d_a, an, x->f, st))) return rc;
+    HIPCHK(hipGetLastError());
+    x->held = an;
+    return NNH_OK;
+}
+
+// The save rule of NTT_info::save (ntt_info.h:237-248): a product of
+// size = an + bn words whose size - 1 exceeds the transform comes out modulo
+// B^L - 1 (L words), otherwise exactly (size words).
+static int xf_save(Device& D, const Plan& pl, u64 size, u64* d_dst, uint64_t* n_out, hipStream_t st,
+                   const std::function<int(u64)>& product) {
+    const bool cyclic = size - 1 > pl.L;
+    int rc = product(cyclic ? pl.L + 2 : size);
+    if (rc) return rc;
+    if (cyclic) {
+        ProfScope ps(K_FOLD, 16 * pl.L, st);
+        launch_cyclic_fold(d_dst, pl.L, st);
+    }
+    HIPCHK(hipGetLastError());
+    if (n_out) *n_out = cyclic ? pl.L : size;
+    return NNH_OK;
+}
+
+int nnh_xf_mul(uint64_t* d_dst, uint64_t* n_out, const nnh_xf* xa, const uint64_t* d_b, uint64_t bn, void* stream) {
+    if (!d_dst || !xa || !xa->held || !d_b || !bn || bn > (1ull << xa->scale))
+        return fail(NNH_ERR_ARG, "bad transform-product arguments");
+    ENTER(stream ? (hipStream_t)stream : g_dev.stream);
+    Device& D = g_dev;
+    Plan pl = make_plan(D, xa->scale);
+    int rc;
+    if ((rc = ensure_twiddles(D, xa->scale)) || (rc = ensure(D.ws, 6 * pl.L))) return rc;
+    // the kept transform plays operand B of the product
+    return xf_save(D, pl, xa->held + bn, d_dst, n_out, st, [&](u64 n) {
+        return product_core(D, pl, d_b, bn, nullptr, xa->held, false, xa->f, d_dst, n, nullptr, 0, st);
+    });
+}
+
+int nnh_xf_mul2(uint64_t* d_dst, uint64_t* n_out, const nnh_xf* xa, const nnh_xf* xb, void* stream) {
+    if (!d_dst || !xa || !xb || !xa->held || !xb->held || xa->scale != xb->scale)
+        return fail(NNH_ERR_ARG, "bad transform-product arguments");
+    ENTER(stream ? (hipStream_t)stream : g_dev.stream);
+    Device& D = g_dev;
+    Plan pl = make_plan(D, xa->scale);
+    int rc;
+    if ((rc = ensure_twiddles(D, xa->scale)) || (rc = ensure(D.ws, 3 * pl.L))) return rc;
+    return xf_save(D, pl, xa->held + xb->held, d_dst, n_out, st, [&](u64 n) {
+        return product_transformed(D, pl, xa->f, xa->held, xb->f, xb->held, false, d_dst, n, st);
+    });
+}
+
+int nnh_xf_sqr(uint64_t* d_dst, uint64_t* n_out, const nnh_xf* xa, void* stream) {
+    if (!d_dst || !xa || !xa->held) return fail(NNH_ERR_ARG, "bad transform-square arguments");
+    ENTER(stream ? (hipStream_t)stream : g_dev.stream);
+    Device& D = g_dev;
+    Plan pl = make_plan(D, xa->scale);
+    int rc;
+    if ((rc = ensure_twiddles(D, xa->scale)) || (rc = ensure(D.ws, 3 * pl.L))) return rc;
+    return xf_save(D, pl, 2 * xa->held, d_dst, n_out, st, [&](u64 n) {
+        return product_transformed(D, pl, xa->f, xa->held, nullptr, xa->held, true, d_dst, n, st);
+    });
 }
 
 }  // extern "C"
