@@ -38,8 +38,11 @@
 #include "nn_oracle.hpp"
 #include "nnh.h"
 
+static int nnh_bit_length(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
 constexpr uint64_t NTTmul_threshold = 384;  // ntt_info.h:6
 constexpr uint64_t NTTdiv_threshold = 256;  // ntt_info.h:7
+constexpr int NTTscale_threshold = 10;      // ntt_info.h:8
 
 struct basic_natural {
     uint64_t* data;
@@ -82,7 +85,7 @@ struct basic_natural {
 struct natural : basic_natural {
     natural() { data[0] = 0; }
     natural(uint64_t v) { data[0] = v; }
-    natural(const nno::Nat& n) {
+    natural(const nno::Limbs& n) {
         resize(n.size);
         std::memcpy(data, n.data, n.size * sizeof(uint64_t));
     }
@@ -121,6 +124,78 @@ struct natural : basic_natural {
 
 static std::ostream& print_cpu(std::ostream& os, const natural& n) { return os << nno::to_dec(n.nat()); }
 
+natural& operator+=(natural& a, const natural& b) { return a.set(nno::add(a.nat(), b.nat())); }
+
+// NTT_info (ntt_info.h:10-67) as the binding extends it: the reference's
+// host transform (the oracle's NttInfo) and the device handle.
+struct NTT_info {
+    int NTT_scale = 0;
+    nnh_xf* dev = nullptr;
+    nno::NttInfo cpu;
+    NTT_info() = default;
+    NTT_info(const natural& a, int scale) { load(a, scale); }
+    NTT_info(const NTT_info&) = delete;
+    NTT_info& operator=(const NTT_info&) = delete;
+    ~NTT_info() {
+        if (dev) nnh_xf_destroy(dev);
+    }
+    void load(const natural& a, int scale) {
+        NTT_scale = scale;
+        cpu.load(a.data, a.size, scale);
+    }
+    void NTT() { cpu.forward(); }
+};
+
+int get_NTT_scale(uint64_t s1, uint64_t s2) { return nnh_bit_length(s1 + s2 - 2); }  // natural.h:384-386
+
+static void save_into(natural& d, nno::NttInfo& ni, uint64_t size) {
+    nno::Limbs out;
+    ni.save(out, size);
+    d = natural(out);
+    d.std();
+}
+
+// The reference bodies of the transform-reuse helpers (natural.h:452-515) on
+// the oracle's host transform.
+void mul_save_cpu(natural& d, const natural& a, const natural& b, NTT_info& ni_a) {
+    if (ni_a.NTT_scale < NTTscale_threshold) {
+        d._mul_base(&a, &b);
+        ni_a.NTT_scale = 0;
+        return;
+    }
+    ni_a.load(a, ni_a.NTT_scale);
+    NTT_info ni_b(b, ni_a.NTT_scale);
+    ni_a.NTT();
+    ni_b.NTT();
+    ni_b.cpu.mul(ni_a.cpu, ni_b.cpu);
+    ni_b.cpu.inverse();
+    save_into(d, ni_b.cpu, a.size + b.size);
+}
+void sqr_save_cpu(natural& d, const natural& a, NTT_info& ni_a) {
+    if (ni_a.NTT_scale < NTTscale_threshold) {
+        d._sqr_base(&a);
+        ni_a.NTT_scale = 0;
+        return;
+    }
+    ni_a.load(a, ni_a.NTT_scale);
+    NTT_info ni_b;
+    ni_a.NTT();
+    ni_b.cpu.mul(ni_a.cpu, ni_a.cpu);
+    ni_b.cpu.inverse();
+    save_into(d, ni_b.cpu, a.size * 2);
+}
+void mul_load_cpu(natural& d, const natural& a, const natural& b, const NTT_info& ni_a) {
+    if (ni_a.NTT_scale == 0) {
+        d._mul_base(&a, &b);
+        return;
+    }
+    NTT_info ni_b(b, ni_a.NTT_scale);
+    ni_b.NTT();
+    ni_b.cpu.mul(ni_a.cpu, ni_b.cpu);
+    ni_b.cpu.inverse();
+    save_into(d, ni_b.cpu, a.size + b.size);
+}
+
 // natural.h:537-564, 771-793
 natural& operator*=(natural& a, const natural& b) { return a._mul(&a, &b); }
 natural operator*(const natural& a, const natural& b) {
@@ -155,6 +230,24 @@ natural operator%(const natural& a, const natural& b) {
 }
 
 #include "binding_snippets.inc"
+
+// calc_e as the reference writes it (natural.h:841-857), over the bound helpers.
+void calc_e(natural& num, natural& den, uint64_t begin, uint64_t end) {
+    if (end - begin == 2) {
+        num = natural(end);
+        den = natural(begin + 1);
+        if (begin) den *= natural(begin);
+        return;
+    }
+    natural num1, den1;
+    calc_e(num, den, begin, (begin + end) >> 1);
+    calc_e(num1, den1, (begin + end) >> 1, end);
+    NTT_info ni_den1;
+    ni_den1.NTT_scale = get_NTT_scale(std::max(den.size, num.size), den1.size);
+    mul_save_NTT_info(num, den1, num, ni_den1);
+    mul_load_NTT_info(den, den1, den, ni_den1);
+    num += num1;
+}
 
 namespace {
 
@@ -310,6 +403,63 @@ int main(int argc, char** argv) {
         os << a;
         disarm("operator<<", on_dev);
         check("operator<< decimal (300 words)", os.str() == want);
+    }
+    // transform reuse: mul_save then several mul_load on one kept transform,
+    // a product longer than the transform (cyclic save), sqr_save
+    {
+        natural a = rnd(12, 3000), b = rnd(13, 4000), c = rnd(14, 1000), big = rnd(15, 6000);
+        const int sc = 13;  // 2^13 = 8192 words: a*b and a*c fit, a*big wraps
+        NTT_info ni_a;
+        ni_a.NTT_scale = sc;
+        natural d1, d2, d3, d4, d5;
+        arm();
+        mul_save_NTT_info(d1, a, b, ni_a);  // inject mode: falls back, the transform kept on the host
+        disarm("mul_save", on_dev);
+        mul_load_NTT_info(d5, a, c, ni_a);
+        // a transform kept on the device, then products from it (inject mode:
+        // each fails on the device and transforms a again on the host)
+        NTT_info ni_k;
+        ni_k.NTT_scale = sc;
+        natural d0;
+        mul_save_NTT_info(d0, a, b, ni_k);
+        if (on_dev && !ni_k.dev) {
+            std::printf("FAIL mul_save kept no device transform\n");
+            g_bad++;
+        }
+        arm();
+        mul_load_NTT_info(d2, a, c, ni_k);
+        disarm("mul_load", on_dev);
+        arm();
+        mul_load_NTT_info(d3, a, big, ni_k);
+        disarm("mul_load (cyclic)", on_dev);
+        NTT_info ni_s;
+        ni_s.NTT_scale = sc;
+        arm();
+        sqr_save_NTT_info(d4, c, ni_s);
+        disarm("sqr_save", on_dev);
+        // the cyclic product: a*big mod B^8192 - 1, from the oracle's product
+        nno::Nat p = nno::mul(a.nat(), big.nat());
+        nno::Nat lo(p.data, std::min<uint64_t>(p.size, 8192)), hi;
+        if (p.size > 8192) hi = nno::Nat(p.data + 8192, p.size - 8192);
+        nno::Nat m = nno::add(lo, hi);  // < 2 (B^8192 - 1): one end-around step
+        if (m.size > 8192 || (m.size == 8192 && std::all_of(m.data, m.data + 8192, [](uint64_t w) { return w == ~0ull; }))) {
+            nno::Nat one(1), mod = nno::sub(nno::shl(one, 64 * 8192), one);
+            m = nno::sub(m, mod);
+        }
+        check("mul_save / mul_load / cyclic mul_load / sqr_save (scale 13)",
+              same(d1, nno::mul(a.nat(), b.nat())) && same(d0, nno::mul(a.nat(), b.nat())) &&
+                  same(d2, nno::mul(a.nat(), c.nat())) && same(d5, nno::mul(a.nat(), c.nat())) && same(d3, m) &&
+                  same(d4, nno::sqr(c.nat())));
+    }
+    // calc_e through the bound helpers against the oracle's calc_e
+    {
+        natural num, den;
+        arm();
+        calc_e(num, den, 0, 1 << 14);
+        disarm("calc_e", on_dev);
+        nno::Nat n2, d2;
+        nno::calc_e(n2, d2, 0, 1 << 14);
+        check("calc_e(0, 2^14) over mul_save / mul_load", same(num, n2) && same(den, d2));
     }
     if (g_bad) {
         std::printf("binding_check: %d failures\n", g_bad);

@@ -7,8 +7,10 @@ on the GPU with a failure injected into each operation's device call
 (nnh_debug_fail_next), where the fallback must give identical results.
 Cases: a *= a, a * b and a *= b across the 384-word threshold, sqr,
 a %= b (remainder aliasing the dividend, natural.h:783-787), a / b, a % b,
-a /= b, a /= a, a %= a, a one-word divisor and decimal output, each
-compared with the oracle."""
+a /= b, a /= a, a %= a, a one-word divisor, decimal output, the
+transform-reuse helpers over device transform handles (mul_save, mul_load
+including a cyclic product, sqr_save) and calc_e written as the reference
+writes it over those helpers, each compared with the oracle."""
 import os
 import subprocess
 
