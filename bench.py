@@ -256,6 +256,16 @@ def batch_mix(nn, stream, valu, reps=5, with_cpu=True):
             nn.mul_algo_device_async(*r, nn.ALGO_REFERENCE, 0, sp)
 
     ts = timed(singles, 2)
+    # the same policy batch with the built-in formula's splits for every bucket
+    from natural_number_amd import autotune
+    saved = {sc: autotune.plan(sc, True) for sc in autotune.BATCH_SCALES}
+    for sc in saved:
+        autotune.set_plan(sc, True, 0)
+    bf = nn.Batch(reqs["policy"], graph=True)
+    tf = timed(lambda: bf.run(sp), reps)
+    bf.close()
+    for sc, (cb, passes, tuned) in saved.items():
+        autotune.set_plan(sc, True, cb if tuned else 0, passes if tuned else [])
     bp.close()
     br.close()
     inp.free()
@@ -278,6 +288,8 @@ def batch_mix(nn, stream, valu, reps=5, with_cpu=True):
         "batched_reference_rule_ms": tr,
         "batched_reference_rule_products_per_s": n / (tr * 1e-3),
         "policy_gain_over_reference_rule": tr / tb,
+        "batched_formula_splits_ms": tf,
+        "tuned_splits_gain": tf / tb,
         "per_product_launch_ms": ts,
         "per_product_launch_products_per_s": n / (ts * 1e-3),
         "speedup": ts / tb,
@@ -501,6 +513,39 @@ def valu_ceiling(nn, kernels, steps):
     return out
 
 
+def plans_compare(nn, stream, step, steps, batch_sizes=None):
+    """The level splits in use (the autotuner's table, plans_gfx950.txt)
+    against the built-in formula, same run: the headline product with the
+    formula's split at s = 25, and (batch_sizes) the batched mix with the
+    formula's splits for every bucket."""
+    from natural_number_amd import autotune
+
+    def time_steps(n):
+        nn.synchronize()
+        e0, e1 = nn.Event(), nn.Event()
+        e0.record(stream)
+        for _ in range(n):
+            step()
+        e1.record(stream)
+        nn.synchronize()
+        return e0.elapsed_ms(e1) / n
+
+    cb, passes, tuned = autotune.plan(25)
+    res = {"s25_plan": [cb] + passes, "s25_tuned": tuned}
+    t_tuned = time_steps(steps)
+    if tuned:
+        autotune.set_plan(25, False, 0)
+        f_cb, f_passes, _ = autotune.plan(25)
+        t_formula = time_steps(steps)
+        autotune.set_plan(25, False, cb, passes)
+        res.update({"s25_formula_plan": [f_cb] + f_passes, "s25_tuned_ms": round(t_tuned, 4),
+                    "s25_formula_ms": round(t_formula, 4), "s25_gain": round(t_formula / t_tuned, 4)})
+    res["tuned_scales"] = {k: [s for s in autotune.SINGLE_SCALES if autotune.plan(s, k == "batch")[2]] if k == "single"
+                           else [s for s in autotune.BATCH_SCALES if autotune.plan(s, True)[2]]
+                           for k in ("single", "batch")}
+    return res
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -529,11 +574,27 @@ def main():
                     help="with --cost-model: write the trained network as a policy file")
     ap.add_argument("--policy-held-out", type=int, default=2000,
                     help="held-out requests on which the default run evaluates the installed policy")
+    ap.add_argument("--tune-plans", default=None, metavar="PATH",
+                    help="only run the level-split autotuner over every scale and write its table to PATH")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.cost_model:
         print(json.dumps(cost_model_run(args.grid, args.held_out, args.save_policy)), flush=True)
+        return
+    if args.tune_plans:
+        import natural_number_amd as nn
+        from natural_number_amd import autotune
+
+        if nn.lib().nnh_init(0) != 0:
+            raise SystemExit("nnh_init(0) failed")
+        for kind, scales in (("single", autotune.SINGLE_SCALES), ("batch", autotune.BATCH_SCALES)):
+            for sc in scales:  # start every scale from the formula
+                autotune.set_plan(sc, kind == "batch", 0)
+        t = time.perf_counter()
+        table = autotune.tune_all(log=log)
+        autotune.save(args.tune_plans)
+        print(json.dumps({"plans": table, "seconds": round(time.perf_counter() - t, 1)}), flush=True)
         return
     if args.copy_calibration:
         import natural_number_amd as nn
@@ -614,6 +675,7 @@ def main():
     b_alg = 16 * (an + bn) + 144 * L
 
     valu = valu_ceiling(nn, kernels, args.steps)
+    plans = plans_compare(nn, stream, step, args.steps)
 
     c2 = c2_variants(nn, stream, max(3, args.steps // 4), not args.no_cpu)
     batch = None if args.no_batch else batch_mix(nn, stream, valu, with_cpu=not args.no_cpu)
@@ -675,6 +737,7 @@ def main():
             "frac": b_alg / (ms * 1e-3) / HBM_PEAK,
         },
         "valu_ceiling": valu,
+        "plans": plans,
         "kernels": kernels,
         "cpu_baseline": cpu,
         "c2_variants": c2,

@@ -276,7 +276,8 @@ int nnh_kernel_times(int kind, double* total_ms, uint64_t* launches, double* byt
  * bounds them from above. */
 int nnh_butterfly_peak(int prime, uint64_t iters, double* butterflies_per_s);
 
-/* Level split of a transform of 2^scale words per prime (for benchmarks;
+/* Active level split of a transform of 2^scale words for single products
+ * (for benchmarks; the tuned one when installed, else the formula;
  * the counterpart of the reference's cache blocking of NTT/INTT into global
  * levels and 2^20-word chunks, ntt_info.h:187-196, 209-219): the fused
  * chunk kernel's chunk is 2^chunk_bits words (it runs chunk_bits - 1 levels
@@ -284,6 +285,36 @@ int nnh_butterfly_peak(int prime, uint64_t iters, double* butterflies_per_s);
  * pass_levels[i] levels each, in forward order (the inverse runs them in
  * reverse).  Needs no device. */
 int nnh_transform_plan(int scale, int* chunk_bits, int* n_passes, int* pass_levels);
+
+/* Level splits and their autotuner (the counterpart of the reference's
+ * fixed cache blocking of NTT/INTT into global levels and 2^20-word chunks,
+ * ntt_info.h:179-221).  A transform of 2^scale words runs its low
+ * chunk_bits levels in the fused chunk kernel and the rest in n_passes
+ * strided top passes (pass_levels[k] levels each, highest first).  The
+ * library keeps one table for single products and one for batch buckets
+ * (batched = 1); scales without an entry use the built-in formula.
+ * nnh_tune_plans times every candidate split of `scale` (chunks of 2^10 ..
+ * 2^12 words beside the top passes, the fewest passes and one more, several
+ * orders; a product of two 2^(scale-1)-word operands, or a batch of such
+ * products of about 2^22 transform words) over `reps` runs, installs the
+ * fastest and returns up to cap timings (res[0] is the formula's split);
+ * *n receives the number of candidates.  nnh_plans_save/load write and read
+ * the table ("nnh-plans 1 <arch> <compute units>", one line per entry); at
+ * nnh_init the library loads $NNH_PLANS, or plans_gfx950.txt beside it when
+ * it was made on the same kind of device (NNH_PLANS=formula: none).
+ * nnh_plan_get returns 1 when the split comes from the table, 0 for the
+ * formula; nnh_plan_set with chunk_bits = 0 returns a scale to the formula. */
+typedef struct nnh_plan_time {
+    int chunk_bits;
+    int n_passes;
+    int pass_levels[4];
+    float ms;
+} nnh_plan_time;
+int nnh_tune_plans(int scale, int batched, int reps, nnh_plan_time* res, int cap, int* n);
+int nnh_plan_set(int scale, int batched, int chunk_bits, int n_passes, const int* pass_levels);
+int nnh_plan_get(int scale, int batched, int* chunk_bits, int* n_passes, int* pass_levels);
+int nnh_plans_save(const char* path);
+int nnh_plans_load(const char* path);
 
 /* Device plumbing (no counterpart in the reference): device memory, copies,
  * streams and events through the library's own HIP runtime, so that a

@@ -40,8 +40,6 @@ constexpr u64 P[3] = {27ull << 56 | 1, 58ull << 56 | 1, 87ull << 56 | 1};
 // reference's root_base table (ntt_info.h:89, 98, 107); every other entry of
 // that table is a repeated square of it.
 constexpr u64 ROOT_GEN[3] = {11, 2, 310};
-constexpr int TILE_BITS = 12;         // top-pass tile and largest fused chunk: 4096 words
-constexpr int MIN_CHUNK_BITS = 10;    // smallest fused chunk beside top passes (64 threads)
 
 static u64 mulmod(u64 a, u64 b, u64 p) { return (u64)((u128)a * b % p); }
 static u64 powmod(u64 b, u64 e, u64 p) {
@@ -149,6 +147,9 @@ struct Device {
     Buf bws, bjobs, btiles;    // workspace of internal batches (reused across calls)
     // device address of `scratch` per device id (null until nnh_init on it)
     u64* na_ptr[NA_MAX_DEVICES]{};
+    // tuned level splits per scale, for single products [0] and batch
+    // buckets [1] (cb = 0: the formula)
+    LevelPlan tuned[2][MAX_SCALE + 1];
     std::mutex mu;
 };
 
@@ -428,46 +429,38 @@ struct ScratchScope {
 struct Plan {
     int s;
     u64 L;
-    int cb;          // fused chunk bits (mid_bits(s))
+    int cb;          // fused chunk bits
     int ntop;        // top levels (s - cb)
+    LevelPlan lp;    // the split (chunk bits and top passes)
     u64 alpha_m[3];
 };
 
-// Fused chunk bits for a transform of 2^s words.  Above 2^20 words the
-// chunk kernel is VALU-bound and the top passes (at most 8 levels each, one
-// HBM round trip per pass) are HBM-bound with VALU to spare, so the top
-// passes take as many levels as the fewest passes allow, the chunk keeping
-// at least 2^10 words (one wave per work-group): 3.5 -> 3.4 ms for the
-// 2^30-bit product.  Smaller transforms (batch buckets) measured faster
-// with the largest chunk.
-static int mid_bits(int s) {
-    if (s <= 20) return std::min(s, TILE_BITS);
-    const int npass = (s - TILE_BITS + 7) / 8;
-    return std::max(MIN_CHUNK_BITS, s - 8 * npass);
-}
-
-static Plan make_plan(const Device& D, int s) {
+// The level split of a transform of 2^s words: the tuned one for single
+// products or batch buckets when the autotuner installed one, else the
+// formula (formula_plan: the chunk kernel is VALU-bound and the top passes
+// HBM-bound with VALU to spare, so above 2^20 words the top passes take as
+// many levels as the fewest passes allow; 3.5 -> 3.4 ms for the 2^30-bit
+// product).
+static Plan make_plan(const Device& D, int s, bool batched = false) {
     Plan pl;
     pl.s = s;
     pl.L = 1ull << s;
-    pl.cb = mid_bits(s);
+    const LevelPlan& t = D.tuned[batched ? 1 : 0][s];
+    pl.lp = t.cb ? t : formula_plan(s);
+    pl.cb = pl.lp.cb;
     pl.ntop = s - pl.cb;
     for (int i = 0; i < 3; i++) pl.alpha_m[i] = to_mont(alpha_plain(D, i, s), P[i]);
     return pl;
 }
 
-// Top levels split into ceil(ntop/8) passes of nearly equal depth, deeper
-// first: deeper passes have narrower tiles (2^(12-nlev) columns), so
-// balancing keeps every row write at least 2^(12-7) words when ntop > 8.
+// Top passes as (lowest global level, levels), in forward order.
 static std::vector<std::pair<int, int>> top_passes(const Plan& pl) {
-    std::vector<std::pair<int, int>> passes;  // (g_lo, nlev), forward order
-    int npass = (pl.ntop + 7) / 8;
-    int g_hi = pl.s - 1, rem = pl.ntop;
-    for (int k = 0; k < npass; k++) {
-        int nl = (rem + (npass - k) - 1) / (npass - k);
+    std::vector<std::pair<int, int>> passes;
+    int g_hi = pl.s - 1;
+    for (int k = 0; k < pl.lp.np; k++) {
+        const int nl = pl.lp.lev[k];
         passes.push_back({g_hi - nl + 1, nl});
         g_hi -= nl;
-        rem -= nl;
     }
     return passes;
 }
@@ -955,7 +948,7 @@ static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
     if (P.n_school) launch_school_batch(B->d_jobs, B->d_sj, P.n_school, P.ncols, st);
     for (const Bucket& bk : P.buckets) {
         const hipStream_t sa = next_stream(), sb = next_stream();
-        Plan pl = make_plan(D, bk.s);
+        Plan pl = make_plan(D, bk.s, true);
         const BJob* jobs = B->d_jobs + bk.first;
         TopBatchParams tp{};
         for (int i = 0; i < 3; i++) tp.alpha_m[i] = pl.alpha_m[i];
@@ -1150,11 +1143,85 @@ static void default_policy() {
     g_err = keep;
 }
 
-// Device set-up, then (once per process) the default policy.
+// ---------------------------------------------------------------------------
+// Level-split plans: the table make_plan reads, its file form and the
+// autotuner that fills it.
+// ---------------------------------------------------------------------------
+static std::string device_key() {
+    hipDeviceProp_t pr{};
+    if (hipGetDeviceProperties(&pr, g_dev.id) != hipSuccess) return "unknown 0";
+    std::string arch = pr.gcnArchName;
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch = arch.substr(0, colon);
+    return arch + " " + std::to_string(pr.multiProcessorCount);
+}
+
+// File: "nnh-plans 1 <arch> <compute units>", then one line per tuned scale,
+// "single|batch <s> <cb> <np> <lev0> <lev1> <lev2> <lev3>".
+static int load_plans_file(const char* path) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(NNH_ERR_ARG, std::string("cannot open plan file ") + path);
+    char magic[32] = {0}, arch[64] = {0};
+    int ver = 0, cus = 0;
+    LevelPlan t[2][MAX_SCALE + 1];
+    std::string bad;
+    if (std::fscanf(f, "%31s %d %63s %d", magic, &ver, arch, &cus) != 4 || std::string(magic) != "nnh-plans" || ver != 1)
+        bad = "malformed plan file";
+    else if (std::string(arch) + " " + std::to_string(cus) != device_key())
+        bad = std::string("plan file made on another device (") + arch + " " + std::to_string(cus) + ")";
+    char kind[16];
+    int sc;
+    LevelPlan lp;
+    while (bad.empty() && std::fscanf(f, "%15s %d %d %d %d %d %d %d", kind, &sc, &lp.cb, &lp.np, &lp.lev[0], &lp.lev[1],
+                                      &lp.lev[2], &lp.lev[3]) == 8) {
+        const std::string k = kind;
+        if ((k != "single" && k != "batch") || !plan_valid(sc, lp)) bad = "malformed plan file";
+        else t[k == "batch"][sc] = lp;
+    }
+    std::fclose(f);
+    if (!bad.empty()) return fail(NNH_ERR_ARG, bad + " " + path);
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    std::copy(&t[0][0], &t[0][0] + 2 * (MAX_SCALE + 1), &g_dev.tuned[0][0]);
+    return NNH_OK;
+}
+
+static std::string beside_library(const char* name) {
+    Dl_info info{};
+    if (!dladdr((void*)&beside_library, &info) || !info.dli_fname) return "";
+    std::string lib = info.dli_fname;
+    const size_t cut = lib.find_last_of('/');
+    return (cut == std::string::npos ? std::string(".") : lib.substr(0, cut)) + "/" + name;
+}
+
+// At nnh_init: NNH_PLANS=<file> loads that table, NNH_PLANS=formula keeps
+// the formula, otherwise plans_gfx950.txt beside the library is loaded when
+// present and made on this kind of device.
+static void default_plans() {
+    const char* e = std::getenv("NNH_PLANS");
+    std::string path;
+    if (e && *e) {
+        if (std::string(e) == "formula") return;
+        path = e;
+    } else {
+        path = beside_library("plans_gfx950.txt");
+        FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
+        if (!f) return;
+        std::fclose(f);
+    }
+    const std::string keep = g_err;
+    if (load_plans_file(path.c_str()) != NNH_OK) std::fprintf(stderr, "libnnh: %s; using the formula\n", g_err.c_str());
+    g_err = keep;
+}
+
+// Device set-up, then (once per process) the default policy and plans.
 static std::once_flag g_policy_once;
 static int lib_init(int dev) {
     int rc = device_init(dev);
-    if (!rc) std::call_once(g_policy_once, default_policy);
+    if (!rc)
+        std::call_once(g_policy_once, [] {
+            default_policy();
+            default_plans();
+        });
     return rc;
 }
 
@@ -1397,16 +1464,14 @@ int nnh_mul_algo_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an,
 int nnh_transform_plan(int scale, int* chunk_bits, int* n_passes, int* pass_levels) {
     if (scale < 2 || scale > MAX_SCALE || !chunk_bits || !n_passes || !pass_levels)
         return fail(NNH_ERR_ARG, "bad transform-plan arguments");
-    Plan pl;
-    pl.s = scale;
-    pl.L = 1ull << scale;
-    pl.cb = mid_bits(scale);
-    pl.ntop = scale - pl.cb;
-    std::vector<std::pair<int, int>> passes = top_passes(pl);
-    if (passes.size() > 4) return fail(NNH_ERR_ARG, "more than four top passes");
-    *chunk_bits = pl.cb;
-    *n_passes = (int)passes.size();
-    for (size_t i = 0; i < passes.size(); i++) pass_levels[i] = passes[i].second;
+    LevelPlan lp = formula_plan(scale);
+    if (g_inited) {
+        std::lock_guard<std::mutex> g(g_dev.mu);
+        if (g_dev.tuned[0][scale].cb) lp = g_dev.tuned[0][scale];
+    }
+    *chunk_bits = lp.cb;
+    *n_passes = lp.np;
+    for (int i = 0; i < lp.np; i++) pass_levels[i] = lp.lev[i];
     return NNH_OK;
 }
 
@@ -1718,6 +1783,148 @@ int nnh_mul_cyclic(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t
     return NNH_OK;
 }
 
+
+// ---- level-split plans and their autotuner ---------------------------------
+
+int nnh_plan_set(int scale, int batched, int chunk_bits, int n_passes, const int* pass_levels) {
+    if (scale < MIN_MID_BITS || scale > MAX_SCALE || (batched != 0 && batched != 1))
+        return fail(NNH_ERR_ARG, "bad plan scale");
+    LevelPlan lp;
+    if (chunk_bits) {
+        if (n_passes < 0 || n_passes > MAX_PASSES || (n_passes && !pass_levels))
+            return fail(NNH_ERR_ARG, "bad plan");
+        lp.cb = chunk_bits;
+        lp.np = n_passes;
+        for (int k = 0; k < n_passes; k++) lp.lev[k] = pass_levels[k];
+        if (!plan_valid(scale, lp)) return fail(NNH_ERR_ARG, "plan not valid at this scale");
+    }
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    (void)hipDeviceSynchronize();  // queued launches were planned with the previous split
+    g_dev.tuned[batched][scale] = lp;
+    return NNH_OK;
+}
+
+int nnh_plan_get(int scale, int batched, int* chunk_bits, int* n_passes, int* pass_levels) {
+    if (scale < MIN_MID_BITS || scale > MAX_SCALE || (batched != 0 && batched != 1) || !chunk_bits || !n_passes ||
+        !pass_levels)
+        return fail(NNH_ERR_ARG, "bad plan query");
+    LevelPlan lp = formula_plan(scale);
+    int tuned = 0;
+    if (g_inited) {
+        std::lock_guard<std::mutex> g(g_dev.mu);
+        if (g_dev.tuned[batched][scale].cb) {
+            lp = g_dev.tuned[batched][scale];
+            tuned = 1;
+        }
+    }
+    *chunk_bits = lp.cb;
+    *n_passes = lp.np;
+    for (int k = 0; k < MAX_PASSES; k++) pass_levels[k] = k < lp.np ? lp.lev[k] : 0;
+    return tuned;
+}
+
+int nnh_plans_save(const char* path) {
+    if (!path) return fail(NNH_ERR_ARG, "null path");
+    NEED_INIT();
+    std::lock_guard<std::mutex> g(g_dev.mu);
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(NNH_ERR_ARG, std::string("cannot write ") + path);
+    std::fprintf(f, "nnh-plans 1 %s\n", device_key().c_str());
+    for (int b = 0; b < 2; b++)
+        for (int sc = 0; sc <= MAX_SCALE; sc++) {
+            const LevelPlan& lp = g_dev.tuned[b][sc];
+            if (lp.cb)
+                std::fprintf(f, "%s %d %d %d %d %d %d %d\n", b ? "batch" : "single", sc, lp.cb, lp.np, lp.lev[0],
+                             lp.lev[1], lp.lev[2], lp.lev[3]);
+        }
+    std::fclose(f);
+    return NNH_OK;
+}
+
+int nnh_plans_load(const char* path) {
+    if (!path) return fail(NNH_ERR_ARG, "null path");
+    NEED_INIT();
+    return load_plans_file(path);
+}
+
+int nnh_tune_plans(int scale, int batched, int reps, nnh_plan_time* res, int cap, int* n) {
+    if (scale < MIN_MID_BITS || scale > MAX_SCALE || (batched != 0 && batched != 1) || reps < 1 || !n ||
+        (cap > 0 && !res))
+        return fail(NNH_ERR_ARG, "bad tuning arguments");
+    ENTER(g_dev.stream);
+    Device& D = g_dev;
+    const std::vector<LevelPlan> cands = plan_candidates(scale);
+    const u64 L = 1ull << scale, half = L / 2;
+    // batches: as many half-transform products as give about 2^22 words of
+    // transform per run (one bucket of that scale)
+    const u64 count = batched ? std::max<u64>(1, (1ull << 22) >> scale) : 1;
+    const u64 words = count * 2 * L;  // operands (2 half) and output (L) per product, and slack
+    u64* buf = nullptr;
+    HIPCHK(hipMalloc((void**)&buf, words * sizeof(u64)));
+    launch_fill(buf, words, 0x9e3779b97f4a7c15ull, st);
+    const LevelPlan saved = D.tuned[batched][scale];
+    std::vector<float> ms(cands.size(), 0.f);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = NNH_OK;
+    auto finish = [&](int code) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(buf);
+        return code;
+    };
+    if ((rc = ensure_twiddles(D, scale)) || (!batched && (rc = ensure(D.ws, 6 * L)))) return finish(rc);
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return finish(fail(NNH_ERR_HIP, "tuning events"));
+    std::vector<IReq> rq(count);
+    for (u64 i = 0; i < count; i++) {
+        u64* base = buf + i * 2 * L;
+        rq[i] = IReq{base + L, base, half, base + half, half, L, nullptr, 0};
+    }
+    for (size_t c = 0; c < cands.size() && !rc; c++) {
+        D.tuned[batched][scale] = cands[c];
+        float t = 0.f;
+        if (!batched) {
+            const Plan pl = make_plan(D, scale);
+            auto one = [&]() { return product_core(D, pl, rq[0].a, half, rq[0].b, half, false, nullptr, rq[0].dst, L, nullptr, 0, st); };
+            if ((rc = one())) break;
+            if (hipEventRecord(e0, st) != hipSuccess) rc = fail(NNH_ERR_HIP, "tuning event");
+            for (int r = 0; r < reps && !rc; r++) rc = one();
+        } else {
+            nnh_batch* B = nullptr;
+            if ((rc = batch_create(D, rq.data(), count, 1 | 2, &B, ARM_NTT))) break;
+            if (hipGraphLaunch(B->exec, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess)
+                rc = fail(NNH_ERR_HIP, "tuning launch");
+            for (int r = 0; r < reps && !rc; r++)
+                if (hipGraphLaunch(B->exec, st) != hipSuccess) rc = fail(NNH_ERR_HIP, "tuning launch");
+            if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = fail(NNH_ERR_HIP, "tuning event");
+            if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(NNH_ERR_HIP, "tuning event");
+            batch_free(B);
+            if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
+            ms[c] = t / (float)(reps * count);
+            continue;
+        }
+        if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+            rc = fail(NNH_ERR_HIP, "tuning event");
+        if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
+        ms[c] = t / (float)reps;
+    }
+    if (rc) {
+        D.tuned[batched][scale] = saved;
+        return finish(rc);
+    }
+    const size_t best = std::min_element(ms.begin(), ms.end()) - ms.begin();
+    D.tuned[batched][scale] = cands[best];
+    *n = (int)cands.size();
+    for (int c = 0; c < std::min<int>(cap, (int)cands.size()); c++) {
+        res[c].chunk_bits = cands[c].cb;
+        res[c].n_passes = cands[c].np;
+        for (int k = 0; k < 4; k++) res[c].pass_levels[k] = cands[c].lev[k];
+        res[c].ms = ms[c];
+    }
+    return finish(NNH_OK);
+}
 
 // ---- transform handles (mul_save / mul_load / sqr_save, natural.h:452-527) --
 

@@ -9,6 +9,85 @@
 
 namespace nnh {
 
+// ---- level splits -----------------------------------------------------------
+
+static void balance(int s, int cb, int np, bool deeper_first, LevelPlan& p) {
+    p.cb = cb;
+    p.np = np;
+    int rem = s - cb;
+    for (int k = 0; k < np; k++) {
+        const int nl = (rem + (np - k) - 1) / (np - k);
+        p.lev[k] = nl;
+        rem -= nl;
+    }
+    if (!deeper_first) std::reverse(p.lev, p.lev + np);
+}
+
+LevelPlan formula_plan(int s) {
+    LevelPlan p;
+    int cb;
+    if (s <= 20) {
+        cb = std::min(s, TILE_BITS);
+    } else {
+        const int npass = (s - TILE_BITS + 7) / 8;
+        cb = std::max(MIN_CHUNK_BITS, s - 8 * npass);
+    }
+    balance(s, cb, (s - cb + 7) / 8, true, p);
+    return p;
+}
+
+bool plan_valid(int s, const LevelPlan& p) {
+    if (s < MIN_MID_BITS || s > MAX_SCALE || p.cb < MIN_MID_BITS || p.cb > TILE_BITS || p.cb > s) return false;
+    // up to one tile, one chunk; beside top passes, chunks of at least one wave
+    if (s <= TILE_BITS && p.cb != s) return false;
+    if (s > TILE_BITS && p.cb < MIN_CHUNK_BITS) return false;
+    if (p.np < 0 || p.np > MAX_PASSES || (p.np == 0) != (p.cb == s)) return false;
+    int sum = 0;
+    for (int k = 0; k < p.np; k++) {
+        if (p.lev[k] < 1 || p.lev[k] > 8) return false;
+        sum += p.lev[k];
+    }
+    if (sum != s - p.cb) return false;
+    // the lowest pass starts at bit cb; its tile of 2^(12 - lev) columns must fit one stride block
+    return p.np == 0 || p.cb + p.lev[p.np - 1] >= TILE_BITS;
+}
+
+std::vector<LevelPlan> plan_candidates(int s) {
+    std::vector<LevelPlan> out;
+    auto add = [&](const LevelPlan& p) {
+        if (!plan_valid(s, p)) return;
+        for (const LevelPlan& q : out)
+            if (q.cb == p.cb && q.np == p.np && std::equal(q.lev, q.lev + q.np, p.lev)) return;
+        out.push_back(p);
+    };
+    add(formula_plan(s));
+    for (int cb = MIN_CHUNK_BITS; cb <= std::min(s, TILE_BITS); cb++) {
+        const int top = s - cb;
+        const int np0 = (top + 7) / 8;
+        for (int np = np0; np <= std::min(np0 + 1, MAX_PASSES); np++) {
+            if (np == 0 && top) continue;
+            LevelPlan p;
+            balance(s, cb, np, true, p);
+            add(p);
+            balance(s, cb, np, false, p);
+            add(p);
+            // deepest first: 8-level passes, the rest last
+            if (np >= 2) {
+                LevelPlan q;
+                q.cb = cb;
+                q.np = np;
+                int rem = top;
+                for (int k = 0; k < np; k++) {
+                    q.lev[k] = std::min(8, rem - (np - 1 - k));
+                    rem -= q.lev[k];
+                }
+                add(q);
+            }
+        }
+    }
+    return out;
+}
+
 int chunk_scale(int arm, u64 small) { return std::max(ntt_scale(small, small), MIN_MID_BITS) + (arm == ARM_NTT_CHUNK1); }
 
 u64 chunk_pieces(u64 big, u64 small, int sp) {

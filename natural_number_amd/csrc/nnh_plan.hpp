@@ -22,6 +22,34 @@ inline int ntt_scale(u64 s1, u64 s2) {
     return x ? 64 - __builtin_clzll(x) : 0;
 }
 
+// ---- level split of one transform ------------------------------------------
+// A transform of 2^s words runs s - 1 butterfly levels: the fused chunk
+// kernel holds chunks of 2^cb words (its cb - 1 levels of each operand, the
+// base multiply and the inverse levels), and np strided top passes, in
+// forward order the highest levels first, run lev[k] levels each over tiles
+// of 4096 words.  The inverse runs the passes in reverse.
+constexpr int TILE_BITS = 12;       // top-pass tile and largest fused chunk: 4096 words
+constexpr int MIN_CHUNK_BITS = 10;  // smallest chunk the formula gives beside top passes (one wave)
+constexpr int MAX_PASSES = 4;
+struct LevelPlan {
+    int cb = 0;  // 0: no plan (the formula applies)
+    int np = 0;
+    int lev[MAX_PASSES] = {0, 0, 0, 0};
+};
+// The hand-derived split: up to 2^20 words the largest chunk (2^12) and
+// balanced passes; above, as many levels in the (HBM-bound) top passes as
+// the fewest passes allow, the (VALU-bound) chunk keeping at least 2^10 words.
+LevelPlan formula_plan(int s);
+// Whether plan p is one the library runs at scale s: cb = s up to 2^12
+// words; above, 10 <= cb <= 12 (work-groups of at least one wave), at most
+// four passes of 1..8 levels summing to s - cb, and every pass's tile inside
+// one stride block (lowest level + depth >= 12).
+bool plan_valid(int s, const LevelPlan& p);
+// The autotuner's candidates at scale s (all valid, the formula's first):
+// chunk sizes 2^10..2^12 with the fewest passes and one more, the passes
+// balanced with the deeper or the shallower first, or the deepest first.
+std::vector<LevelPlan> plan_candidates(int s);
+
 // Algorithm selectors of nnh_mul_algo (include/nnh.h).
 enum Algo {
     ALGO_AUTO = 0,       // the library's dispatch: the installed policy, else ALGO_REFERENCE

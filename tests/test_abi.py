@@ -75,3 +75,19 @@ def test_cpp_class_header_compiles(tmp_path):
         r = subprocess.run(["g++", f"-std={std}", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
                             "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_transform_plan_formula():
+    """The built-in level split (no device, no tuned table): up to 2^20 words
+    the largest chunk (2^12) and balanced passes deeper first; above, the
+    fewest passes of at most 8 levels and a chunk of at least 2^10 words."""
+    import ctypes
+
+    import natural_number_amd as nn
+
+    L = nn.lib()
+    cb, np_, lev = ctypes.c_int(), ctypes.c_int(), (ctypes.c_int * 4)()
+    for s, want in [(8, (8, [])), (12, (12, [])), (13, (12, [1])), (20, (12, [8])), (21, (10, [6, 5])),
+                    (25, (10, [8, 7])), (28, (12, [8, 8])), (30, (10, [7, 7, 6]))]:
+        assert L.nnh_transform_plan(s, ctypes.byref(cb), ctypes.byref(np_), lev) == 0
+        assert (cb.value, list(lev)[:np_.value]) == want, s
