@@ -574,6 +574,8 @@ def main():
                     help="with --cost-model: write the trained network as a policy file")
     ap.add_argument("--policy-held-out", type=int, default=2000,
                     help="held-out requests on which the default run evaluates the installed policy")
+    ap.add_argument("--batch-profile", type=int, default=0, metavar="RUNS",
+                    help="only run the 4096-product mix as an eager batch (no graph) RUNS times, for counter profiles")
     ap.add_argument("--tune-plans", default=None, metavar="PATH",
                     help="only run the level-split autotuner over every scale and write its table to PATH")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -582,17 +584,53 @@ def main():
     if args.cost_model:
         print(json.dumps(cost_model_run(args.grid, args.held_out, args.save_policy)), flush=True)
         return
+    if args.batch_profile:
+        import natural_number_amd as nn
+        from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
+
+        if nn.lib().nnh_init(0) != 0:
+            raise SystemExit("nnh_init(0) failed")
+        sizes = batch_mix_sizes()
+        host = np.concatenate([np.concatenate([a, b]) for a, b in batch_mix_operands(sizes)])
+        inp = nn.DeviceWords(host.size, host)
+        out = nn.DeviceWords(sum(a + b for a, b in sizes))
+        reqs, ia, oa = [], 0, 0
+        for an, bn in sizes:
+            reqs.append((out.at(oa), inp.at(ia), an, inp.at(ia + an), bn))
+            ia += an + bn
+            oa += an + bn
+        b = nn.Batch(reqs, graph=False)
+        for k in range(args.batch_profile):
+            b.run()
+            nn.synchronize()
+            log(f"batch run {k + 1}/{args.batch_profile}")
+        b.close()
+        print(json.dumps({"batch_profile_runs": args.batch_profile, "products": len(sizes)}), flush=True)
+        return
     if args.tune_plans:
         import natural_number_amd as nn
         from natural_number_amd import autotune
 
         if nn.lib().nnh_init(0) != 0:
             raise SystemExit("nnh_init(0) failed")
+        from natural_number_amd.workloads import batch_mix_operands, batch_mix_sizes
+
         for kind, scales in (("single", autotune.SINGLE_SCALES), ("batch", autotune.BATCH_SCALES)):
             for sc in scales:  # start every scale from the formula
                 autotune.set_plan(sc, kind == "batch", 0)
         t = time.perf_counter()
         table = autotune.tune_all(log=log)
+        # batch buckets: tuned on the 4096-product uneven mix as one batch
+        sizes = batch_mix_sizes()
+        host = np.concatenate([np.concatenate([a, b]) for a, b in batch_mix_operands(sizes)])
+        inp = nn.DeviceWords(host.size, host)
+        out = nn.DeviceWords(sum(a + b for a, b in sizes))
+        reqs, ia, oa = [], 0, 0
+        for an, bn in sizes:
+            reqs.append((out.at(oa), inp.at(ia), an, inp.at(ia + an), bn))
+            ia += an + bn
+            oa += an + bn
+        table["batch_workload"] = autotune.tune_batch_workload(reqs, log=log)
         autotune.save(args.tune_plans)
         print(json.dumps({"plans": table, "seconds": round(time.perf_counter() - t, 1)}), flush=True)
         return

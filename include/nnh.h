@@ -295,10 +295,14 @@ int nnh_transform_plan(int scale, int* chunk_bits, int* n_passes, int* pass_leve
  * (batched = 1); scales without an entry use the built-in formula.
  * nnh_tune_plans times every candidate split of `scale` (chunks of 2^10 ..
  * 2^12 words beside the top passes, the fewest passes and one more, several
- * orders; a product of two 2^(scale-1)-word operands, or a batch of such
- * products of about 2^22 transform words) over `reps` runs, installs the
- * fastest and returns up to cap timings (res[0] is the formula's split);
- * *n receives the number of candidates.  nnh_plans_save/load write and read
+ * orders) on a product of two 2^(scale-1)-word operands, or on a batch of
+ * such products of about 2^22 transform words: after about 100 ms of
+ * warm-up runs, three rounds time every candidate in turn over `reps` runs
+ * and each keeps its fastest round; the fastest candidate is installed when
+ * it beats the formula's split by more than 1 %, else the scale keeps the
+ * formula.  Up to cap timings are returned (res[0] is the formula's split)
+ * and *n receives the number of candidates; nnh_plan_candidates lists the
+ * candidates without timing them (ms = 0).  nnh_plans_save/load write and read
  * the table ("nnh-plans 1 <arch> <compute units>", one line per entry); at
  * nnh_init the library loads $NNH_PLANS, or plans_gfx950.txt beside it when
  * it was made on the same kind of device (NNH_PLANS=formula: none).
@@ -311,6 +315,7 @@ typedef struct nnh_plan_time {
     float ms;
 } nnh_plan_time;
 int nnh_tune_plans(int scale, int batched, int reps, nnh_plan_time* res, int cap, int* n);
+int nnh_plan_candidates(int scale, nnh_plan_time* res, int cap, int* n);
 int nnh_plan_set(int scale, int batched, int chunk_bits, int n_passes, const int* pass_levels);
 int nnh_plan_get(int scale, int batched, int* chunk_bits, int* n_passes, int* pass_levels);
 int nnh_plans_save(const char* path);

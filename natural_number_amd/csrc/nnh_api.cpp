@@ -1848,74 +1848,107 @@ int nnh_plans_load(const char* path) {
     return load_plans_file(path);
 }
 
+int nnh_plan_candidates(int scale, nnh_plan_time* res, int cap, int* n) {
+    if (scale < MIN_MID_BITS || scale > MAX_SCALE || !n || (cap > 0 && !res)) return fail(NNH_ERR_ARG, "bad arguments");
+    const std::vector<LevelPlan> cands = plan_candidates(scale);
+    *n = (int)cands.size();
+    for (int c = 0; c < std::min<int>(cap, (int)cands.size()); c++) {
+        res[c] = nnh_plan_time{cands[c].cb, cands[c].np, {cands[c].lev[0], cands[c].lev[1], cands[c].lev[2], cands[c].lev[3]},
+                               0.f};
+    }
+    return NNH_OK;
+}
+
+// Single products: every candidate split of `scale` timed on a product of
+// two 2^(scale-1)-word operands, each product followed by an event record
+// as the entry points do.  The device first runs products for about 100 ms
+// (clocks settle), then `rounds` rounds time every candidate in turn over
+// `reps` products, and each candidate keeps its fastest round.  A candidate
+// replaces the formula only when it is faster by more than `margin`.
+// Batch buckets (batched = 1): a batch of about 2^22 transform words of
+// such products per run, replayed as a graph.
 int nnh_tune_plans(int scale, int batched, int reps, nnh_plan_time* res, int cap, int* n) {
     if (scale < MIN_MID_BITS || scale > MAX_SCALE || (batched != 0 && batched != 1) || reps < 1 || !n ||
         (cap > 0 && !res))
         return fail(NNH_ERR_ARG, "bad tuning arguments");
     ENTER(g_dev.stream);
     Device& D = g_dev;
+    constexpr int ROUNDS = 3;
+    constexpr float MARGIN = 0.01f;
     const std::vector<LevelPlan> cands = plan_candidates(scale);
     const u64 L = 1ull << scale, half = L / 2;
-    // batches: as many half-transform products as give about 2^22 words of
-    // transform per run (one bucket of that scale)
     const u64 count = batched ? std::max<u64>(1, (1ull << 22) >> scale) : 1;
-    const u64 words = count * 2 * L;  // operands (2 half) and output (L) per product, and slack
+    const u64 words = count * 2 * L;  // operands (2 half) and output (L) per product
     u64* buf = nullptr;
     HIPCHK(hipMalloc((void**)&buf, words * sizeof(u64)));
-    launch_fill(buf, words, 0x9e3779b97f4a7c15ull, st);
+    // random operands: the power drawn (and so the clock) depends on the bits switched
+    launch_fill_hash(buf, words, 0x4E4E2E19ull + (u64)scale, st);
     const LevelPlan saved = D.tuned[batched][scale];
     std::vector<float> ms(cands.size(), 0.f);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, ec = nullptr;
     int rc = NNH_OK;
+    std::vector<nnh_batch*> graphs(cands.size(), nullptr);
     auto finish = [&](int code) {
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
+        for (nnh_batch* B : graphs) batch_free(B);
+        for (hipEvent_t e : {e0, e1, ec})
+            if (e) (void)hipEventDestroy(e);
         (void)hipStreamSynchronize(st);
         (void)hipFree(buf);
+        if (code) D.tuned[batched][scale] = saved;
         return code;
     };
     if ((rc = ensure_twiddles(D, scale)) || (!batched && (rc = ensure(D.ws, 6 * L)))) return finish(rc);
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipEventCreateWithFlags(&ec, hipEventDisableTiming) != hipSuccess)
         return finish(fail(NNH_ERR_HIP, "tuning events"));
     std::vector<IReq> rq(count);
     for (u64 i = 0; i < count; i++) {
         u64* base = buf + i * 2 * L;
         rq[i] = IReq{base + L, base, half, base + half, half, L, nullptr, 0};
     }
-    for (size_t c = 0; c < cands.size() && !rc; c++) {
-        D.tuned[batched][scale] = cands[c];
-        float t = 0.f;
-        if (!batched) {
-            const Plan pl = make_plan(D, scale);
-            auto one = [&]() { return product_core(D, pl, rq[0].a, half, rq[0].b, half, false, nullptr, rq[0].dst, L, nullptr, 0, st); };
-            if ((rc = one())) break;
-            if (hipEventRecord(e0, st) != hipSuccess) rc = fail(NNH_ERR_HIP, "tuning event");
-            for (int r = 0; r < reps && !rc; r++) rc = one();
-        } else {
-            nnh_batch* B = nullptr;
-            if ((rc = batch_create(D, rq.data(), count, 1 | 2, &B, ARM_NTT))) break;
-            if (hipGraphLaunch(B->exec, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess)
-                rc = fail(NNH_ERR_HIP, "tuning launch");
-            for (int r = 0; r < reps && !rc; r++)
-                if (hipGraphLaunch(B->exec, st) != hipSuccess) rc = fail(NNH_ERR_HIP, "tuning launch");
-            if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = fail(NNH_ERR_HIP, "tuning event");
-            if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(NNH_ERR_HIP, "tuning event");
-            batch_free(B);
-            if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
-            ms[c] = t / (float)(reps * count);
-            continue;
+    if (batched)  // one graph (with its own buffers) per candidate, planned while its split is installed
+        for (size_t c = 0; c < cands.size() && !rc; c++) {
+            D.tuned[1][scale] = cands[c];
+            rc = batch_create(D, rq.data(), count, 1, &graphs[c], ARM_NTT);
         }
-        if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
-            rc = fail(NNH_ERR_HIP, "tuning event");
-        if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
-        ms[c] = t / (float)reps;
+    if (rc) return finish(rc);
+    // one run of candidate c: a product or a graph replay
+    auto run = [&](size_t c) -> int {
+        if (batched) return hipGraphLaunch(graphs[c]->exec, st) == hipSuccess ? NNH_OK : fail(NNH_ERR_HIP, "tuning launch");
+        D.tuned[0][scale] = cands[c];
+        const Plan pl = make_plan(D, scale);
+        int r = product_core(D, pl, rq[0].a, half, rq[0].b, half, false, nullptr, rq[0].dst, L, nullptr, 0, st);
+        if (!r && hipEventRecord(ec, st) != hipSuccess) r = fail(NNH_ERR_HIP, "tuning event");
+        return r;
+    };
+    // settle the clocks: about 100 ms of the formula's runs
+    {
+        float t = 0.f;
+        if ((rc = run(0)) || hipEventRecord(e0, st) != hipSuccess || (rc = run(0)) || hipEventRecord(e1, st) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess)
+            return finish(rc ? rc : fail(NNH_ERR_HIP, "tuning event"));
+        (void)hipEventElapsedTime(&t, e0, e1);
+        const int warm = (int)std::min(2000.f, 100.f / std::max(t, 1e-3f));
+        for (int w = 0; w < warm && !rc; w++) rc = run(0);
+        if (rc) return finish(rc);
     }
-    if (rc) {
-        D.tuned[batched][scale] = saved;
-        return finish(rc);
-    }
-    const size_t best = std::min_element(ms.begin(), ms.end()) - ms.begin();
-    D.tuned[batched][scale] = cands[best];
+    for (int round = 0; round < ROUNDS && !rc; round++)
+        for (size_t c = 0; c < cands.size() && !rc; c++) {
+            float t = 0.f;
+            if ((rc = run(c))) break;  // one untimed run with this split first
+            if (hipEventRecord(e0, st) != hipSuccess) rc = fail(NNH_ERR_HIP, "tuning event");
+            for (int r = 0; r < reps && !rc; r++) rc = run(c);
+            if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                rc = fail(NNH_ERR_HIP, "tuning event");
+            if (rc) break;
+            (void)hipEventElapsedTime(&t, e0, e1);
+            t /= (float)(reps * count);
+            ms[c] = round == 0 ? t : std::min(ms[c], t);
+        }
+    if (rc) return finish(rc);
+    size_t best = std::min_element(ms.begin(), ms.end()) - ms.begin();
+    if (ms[best] > ms[0] * (1.f - MARGIN)) best = 0;
+    D.tuned[batched][scale] = best == 0 ? LevelPlan{} : cands[best];
     *n = (int)cands.size();
     for (int c = 0; c < std::min<int>(cap, (int)cands.size()); c++) {
         res[c].chunk_bits = cands[c].cb;

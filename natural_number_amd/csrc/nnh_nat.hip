@@ -2,6 +2,8 @@
 // modular add/sub with tile carry functions, shifts, complement, searches,
 // and the kernels of the binary-splitting and radix-conversion workloads
 // (reference natural.h:841-857 and 904-1039).
+#include <algorithm>
+
 #include "nnh_carry.hpp"
 #include "nnh_nat.hpp"
 #include "nnh_tile.hpp"
@@ -123,6 +125,15 @@ void launch_add_one(AddJob job, u8* d_tiles, hipStream_t st) {
 __global__ void k_not(u64* out, const u64* a, u64 n) {
     for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) out[k] = ~a[k];
 }
+// out[k] = splitmix64 output for state seed + (k + 1) * gamma (random-looking words)
+__global__ void k_fill_hash(u64* out, u64 n, u64 seed) {
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) {
+        u64 z = seed + (k + 1) * 0x9e3779b97f4a7c15ull;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        out[k] = z ^ (z >> 31);
+    }
+}
 __global__ void k_fill(u64* out, u64 n, u64 v) {
     for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) out[k] = v;
 }
@@ -198,6 +209,9 @@ void launch_not(u64* out, const u64* a, u64 n, hipStream_t st) {
 }
 void launch_fill(u64* out, u64 n, u64 v, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, st, out, n, v);
+}
+void launch_fill_hash(u64* out, u64 n, u64 seed, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_hash, dim3((unsigned)std::min<u64>((n + 255) / 256, 8192)), dim3(256), 0, st, out, n, seed);
 }
 void launch_shl_bits(u64* out, const u64* a, u64 n, u32 b, hipStream_t st) {
     hipLaunchKernelGGL(k_shl_bits, dim3(grid_for(n + 1)), dim3(256), 0, st, out, a, n, b);
@@ -626,21 +640,124 @@ __device__ __forceinline__ u64 div_1e18(u64& r, u64 u) {
 }
 
 // q[0..n) = a / d0, *rem = a mod d0 for a one-word divisor (reference
-// div_base, natural.h:566-574); d = d0 << l normalised.  A single thread
-// walks the words from the top (the remainder chain is sequential).
+// div_base, natural.h:566-574); d = d0 << l normalised, the dividend taken
+// as a << l (its words u_k below, plus a top remainder of the bits shifted
+// out).  The remainder chain runs top-down, one word after another; across
+// chunks of DIVW_CHUNK words it is an affine map r -> (r B^len + rho) mod d
+// of the remainder entering the chunk (B = 2^64, rho the chunk's own value
+// mod d), and affine maps compose.  So: every chunk's rho in parallel
+// (k_divw_local), the remainder entering every chunk by a scan of the
+// composed maps in one work-group (k_divw_scan), then every chunk divides
+// its words in parallel from that remainder (k_divw_final).
+constexpr u64 DIVW_CHUNK = 64;
+constexpr u64 DIVW_SERIAL_MAX = 2048;  // up to this many words one thread does it all
+
+__device__ __forceinline__ u64 shifted_word(const u64* a, u64 k, u32 l) {
+    u64 u = a[k] << l;
+    if (l && k > 0) u |= a[k - 1] >> (64 - l);
+    return u;
+}
+__device__ __forceinline__ u64 mulmod_pre(u64 x, u64 y, u64 d, u64 v) {  // x, y < d
+    u64 lo, hi, r;
+    mul_wide(x, y, lo, hi);
+    (void)div_preinv(hi, lo, d, v, r);
+    return r;
+}
+__device__ __forceinline__ u64 addmod_d(u64 x, u64 y, u64 d) {  // x, y < d, d >= 2^63
+    const u64 s = x + y;
+    return (s < x || s >= d) ? s - d : s;
+}
+
 __global__ void k_div_word(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64* rem) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     u64 r = l ? a[n - 1] >> (64 - l) : 0;
-    for (u64 k = n; k-- > 0;) {
-        u64 u0 = a[k] << l;
-        if (l && k > 0) u0 |= a[k - 1] >> (64 - l);
-        q[k] = div_preinv(r, u0, d, v, r);
-    }
+    for (u64 k = n; k-- > 0;) q[k] = div_preinv(r, shifted_word(a, k, l), d, v, r);
     *rem = r >> l;
 }
 
-void launch_div_word(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64* rem, hipStream_t st) {
-    hipLaunchKernelGGL(k_div_word, dim3(1), dim3(64), 0, st, q, a, n, d, l, v, rem);
+__global__ void k_divw_local(const u64* a, u64 n, u64 d, u32 l, u64 v, u64 nch, u64* rho) {
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nch; j += (u64)gridDim.x * blockDim.x) {
+        const u64 lo = j * DIVW_CHUNK, hi = lo + DIVW_CHUNK < n ? lo + DIVW_CHUNK : n;
+        u64 r = 0;
+        for (u64 k = hi; k-- > lo;) (void)div_preinv(r, shifted_word(a, k, l), d, v, r);
+        rho[j] = r;
+    }
+}
+
+// R[j] = the remainder entering chunk j.  beta = B^DIVW_CHUNK mod d,
+// beta_top = B^(length of the top chunk) mod d.
+__global__ __launch_bounds__(1024) void k_divw_scan(const u64* a, u64 n, u32 l, const u64* rho, u64 nch, u64 beta,
+                                                    u64 beta_top, u64 d, u64 v, u64* R) {
+    __shared__ u64 sm[2][1024], sc[2][1024];
+    const u32 t = threadIdx.x;
+    const u64 seg = (nch + 1023) / 1024;
+    const u64 lo = t * seg < nch ? t * seg : nch, hi = lo + seg < nch ? lo + seg : nch;
+    // this thread's chunks applied from the top down: r -> r m + c
+    u64 m = 1, c = 0;
+    for (u64 j = hi; j-- > lo;) {
+        const u64 bj = j == nch - 1 ? beta_top : beta;
+        m = mulmod_pre(m, bj, d, v);
+        c = addmod_d(mulmod_pre(c, bj, d, v), rho[j], d);
+    }
+    sm[0][t] = m;
+    sc[0][t] = c;
+    __syncthreads();
+    // suffix composition: after the steps, thread t holds every segment at or
+    // above t applied from the top one down (its own last)
+    int cur = 0;
+    for (u32 off = 1; off < 1024; off <<= 1) {
+        u64 mm = sm[cur][t], cc = sc[cur][t];
+        if (t + off < 1024) {
+            const u64 my = sm[cur][t + off], cy = sc[cur][t + off];  // applied first
+            cc = addmod_d(mulmod_pre(cy, mm, d, v), cc, d);
+            mm = mulmod_pre(my, mm, d, v);
+        }
+        sm[cur ^ 1][t] = mm;
+        sc[cur ^ 1][t] = cc;
+        __syncthreads();
+        cur ^= 1;
+    }
+    const u64 r_init = l ? a[n - 1] >> (64 - l) : 0;
+    u64 r = t == 1023 ? r_init : addmod_d(mulmod_pre(r_init, sm[cur][t + 1], d, v), sc[cur][t + 1], d);
+    for (u64 j = hi; j-- > lo;) {
+        R[j] = r;
+        const u64 bj = j == nch - 1 ? beta_top : beta;
+        r = addmod_d(mulmod_pre(r, bj, d, v), rho[j], d);
+    }
+}
+
+__global__ void k_divw_final(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64 nch, const u64* R, u64* rem) {
+    for (u64 j = (u64)blockIdx.x * blockDim.x + threadIdx.x; j < nch; j += (u64)gridDim.x * blockDim.x) {
+        const u64 lo = j * DIVW_CHUNK, hi = lo + DIVW_CHUNK < n ? lo + DIVW_CHUNK : n;
+        u64 r = R[j];
+        for (u64 k = hi; k-- > lo;) q[k] = div_preinv(r, shifted_word(a, k, l), d, v, r);
+        if (j == 0) *rem = r >> l;
+    }
+}
+
+u64 div_word_scratch_words(u64 n) { return n <= DIVW_SERIAL_MAX ? 1 : 2 * ((n + DIVW_CHUNK - 1) / DIVW_CHUNK); }
+
+void launch_div_word(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64* rem, u64* scratch, hipStream_t st) {
+    if (n <= DIVW_SERIAL_MAX) {
+        hipLaunchKernelGGL(k_div_word, dim3(1), dim3(64), 0, st, q, a, n, d, l, v, rem);
+        return;
+    }
+    const u64 nch = (n + DIVW_CHUNK - 1) / DIVW_CHUNK, top_len = n - (nch - 1) * DIVW_CHUNK;
+    // B mod d = 2^64 - d (d >= 2^63), and its powers
+    typedef unsigned __int128 u128;
+    auto powmod = [&](u64 e) {
+        u64 b = 0 - d, r = 1;
+        for (; e; e >>= 1, b = (u64)((u128)b * b % d))
+            if (e & 1) r = (u64)((u128)r * b % d);
+        return r;
+    };
+    u64* rho = scratch;
+    u64* R = scratch + nch;
+    const unsigned blocks = (unsigned)std::min<u64>((nch + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_divw_local, dim3(blocks), dim3(256), 0, st, a, n, d, l, v, nch, rho);
+    hipLaunchKernelGGL(k_divw_scan, dim3(1), dim3(1024), 0, st, a, n, l, (const u64*)rho, nch, powmod(DIVW_CHUNK),
+                       powmod(top_len), d, v, R);
+    hipLaunchKernelGGL(k_divw_final, dim3(blocks), dim3(256), 0, st, q, a, n, d, l, v, nch, (const u64*)R, rem);
 }
 
 __device__ __forceinline__ void put9(char* o, u32 v) {

@@ -14,6 +14,8 @@ void launch_add_one(AddJob job, u8* d_tiles, hipStream_t st);
 
 void launch_not(u64* out, const u64* a, u64 n, hipStream_t st);
 void launch_fill(u64* out, u64 n, u64 v, hipStream_t st);
+// out[k] = splitmix64(seed + (k + 1) gamma): random-looking words (timing inputs).
+void launch_fill_hash(u64* out, u64 n, u64 seed, hipStream_t st);
 // out[0..n] = a << b (1 <= b <= 63): n + 1 words.
 void launch_shl_bits(u64* out, const u64* a, u64 n, u32 b, hipStream_t st);
 // out[0..n) = a >> b (1 <= b <= 63).
@@ -27,8 +29,10 @@ void launch_find(const u64* a, const u64* b, u64 n, int mode, u64* res, u64* hou
 void launch_peek(const u64* p, u32 n, u64* hout, u64 seq, hipStream_t st);
 void launch_set_word(u64* p, u64 v, hipStream_t st);
 // q[0..n) = a / d0 and *rem = a mod d0, where d = d0 << l has its top bit
-// set and v = floor((2^128 - 1) / d) - 2^64.
-void launch_div_word(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64* rem, hipStream_t st);
+// set and v = floor((2^128 - 1) / d) - 2^64; scratch holds
+// div_word_scratch_words(n) words (the chunks' remainders).
+u64 div_word_scratch_words(u64 n);
+void launch_div_word(u64* q, const u64* a, u64 n, u64 d, u32 l, u64 v, u64* rem, u64* scratch, hipStream_t st);
 
 // Binary splitting of sum 1/k! (reference calc_e, natural.h:841-857).
 // Leaves: thread j folds the 16 pairs [32j, 32j+32) into (num, den) at

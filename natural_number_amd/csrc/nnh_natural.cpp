@@ -613,7 +613,8 @@ void divmod(Ctx& c, const DN& a, const DN& b, DN& q, DN& r) {
         u64 v = (u64)((~(u128)0) / d - ((u128)1 << 64));
         q = c.alloc(a.n);
         DN rem = c.alloc(1);
-        launch_div_word(q.p, a.p, a.n, d, l, v, rem.p, c.st);
+        DN scr = c.alloc(div_word_scratch_words(a.n));
+        launch_div_word(q.p, a.p, a.n, d, l, v, rem.p, scr.p, c.st);
         c.normalize(q);
         r = rem;
         return;

@@ -240,3 +240,17 @@ def test_shutdown_releases_and_recovers(nnh, oracle):
     assert after == before
     assert after[0] == oracle.e_digits(12, 3000) and after[1] == oracle.to_dec(a)
     assert after[2] == to_int(a) ** 2
+
+
+@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, 4097, 65537, 1000000])
+def test_divmod_one_word_divisor_vs_python(nnh, n):
+    """One-word divisors (div_base, natural.h:566-574): the chunked
+    remainder scan above 2048 words against Python's integers, divisors with
+    and without the top bit, 1 and 2^64 - 1."""
+    rng = np.random.default_rng(n)
+    a = rand_limbs(rng, n)
+    A = to_int(a)
+    for d in (1, 3, 10**18, (1 << 63) + 12345, (1 << 64) - 1, int(rng.integers(2, 2**62))):
+        q, r = nnh.divmod_(a, np.array([d], dtype=np.uint64))
+        qq, rr = divmod(A, d)
+        assert to_int(q) == qq and to_int(r) == rr, (n, d)

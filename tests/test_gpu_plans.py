@@ -78,7 +78,10 @@ def test_tuner_times_and_installs(nnh, at):
         assert len(r) >= 3 and all(x["ms"] > 0 for x in r)
         best = min(r, key=lambda x: x["ms"])
         cb, passes, tuned = at_.plan(s, batched)
-        assert tuned and [cb] + passes == [best["chunk_bits"]] + best["passes"]
+        if best["ms"] < 0.99 * r[0]["ms"]:  # installed only when it beats the formula by more than 1 %
+            assert tuned and [cb] + passes == [best["chunk_bits"]] + best["passes"]
+        else:
+            assert not tuned
 
 
 def test_plan_file_round_trip(nnh, at, tmp_path):
