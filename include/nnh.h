@@ -150,8 +150,12 @@ int nnh_arm_call(int arm, uint64_t an, uint64_t bn, int* algo, int* param);
 
 /* Device-pointer variants: operands and result already in HBM, work is
  * queued on `stream` (a hipStream_t, NULL = the library's stream) and the
- * call returns without synchronising.  d_dst must not overlap d_a or d_b
- * (the split path writes its first half before loading the second). */
+ * call returns without synchronising, except when the dispatch (the
+ * installed cost-model policy) takes the chunked arm, which plans its
+ * pieces as a batch and synchronises the stream.  d_dst must not overlap
+ * d_a or d_b (the split path writes its first half before loading the
+ * second).  Calls on different streams are ordered by the library (each
+ * waits for the previous call's work in the shared workspace). */
 int nnh_mul_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an,
                          const uint64_t* d_b, uint64_t bn, void* stream);
 int nnh_sqr_device_async(uint64_t* d_dst, const uint64_t* d_a, uint64_t an, void* stream);
