@@ -23,6 +23,14 @@ __host__ __device__ __forceinline__ u32 lo32(u64 x) { return (u32)x; }
 __host__ __device__ __forceinline__ u32 hi32(u64 x) { return (u32)(x >> 32); }
 __host__ __device__ __forceinline__ u64 mk64(u32 lo, u32 hi) { return ((u64)hi << 32) | lo; }
 
+// Residue words are read once and written once per pass: the transform
+// passes move them with nontemporal loads and stores, so that the streamed
+// data does not evict the twiddle tables every work-group re-reads from
+// the caches (same-box A/B on the 2^30-bit product: 3.53 -> 3.38 ms).
+// Twiddles and limbs keep ordinary loads.
+__device__ __forceinline__ u64 ld_stream(const u64* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(u64* p, u64 v) { __builtin_nontemporal_store(v, p); }
+
 // Full 64x64 -> 128-bit product from four 32x32+64 multiply-adds.
 __host__ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi) {
     u32 a0 = lo32(a), a1 = hi32(a), b0 = lo32(b), b1 = hi32(b);

@@ -40,7 +40,7 @@ struct CrtIn {
 };
 __device__ __forceinline__ CrtIn crt_load(const CrtArgs& A, u64 k) {
     if (k >= A.n_res) return CrtIn{0, 0, 0};
-    return CrtIn{A.r[0][k], A.r[1][k], A.r[2][k]};
+    return CrtIn{ld_stream(&A.r[0][k]), ld_stream(&A.r[1][k]), ld_stream(&A.r[2][k])};
 }
 __device__ __forceinline__ void crt_finish(const CrtArgs& A, u64 k, const CrtIn& in, u64& x0, u64& x1, u64& x2) {
     if (k >= A.n_res) x0 = x1 = x2 = 0;

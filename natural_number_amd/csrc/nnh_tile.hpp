@@ -229,7 +229,7 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
             else if (top_copy) load_limbs<8>(P, v, A.limbs, A.an, alpha, xs);
             else load_limbs<16>(P, v, A.limbs, A.an, alpha, xs);
         } else {
-            CFor<0, 16>::run([&](auto jc) { v[decltype(jc)::value] = data[xs(decltype(jc)::value)]; });
+            CFor<0, 16>::run([&](auto jc) { v[decltype(jc)::value] = ld_stream(&data[xs(decltype(jc)::value)]); });
         }
     }
     if constexpr (!INV) {
@@ -250,7 +250,7 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
             constexpr u32 jy = (u32)decltype(jc)::value << LO_ST;
             const u32 y = yt | jy;
             const u64 x = xbase + (y & QM) + (u64)(y >> Q) * d;
-            data[x] = v[decltype(jc)::value];
+            st_stream(&data[x], v[decltype(jc)::value]);
         });
     }
 }
@@ -308,14 +308,14 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
         if (A.an == 0) CFor<0, 16>::run([&](auto jc) { a[decltype(jc)::value] = 0; });
         else load_limbs<16>(P, a, A.la, A.an, alpha_(), xs);
     } else {
-        CFor<0, 16>::run([&](auto jc) { a[decltype(jc)::value] = fa_()[xs(decltype(jc)::value)]; });
+        CFor<0, 16>::run([&](auto jc) { a[decltype(jc)::value] = ld_stream(&fa_()[xs(decltype(jc)::value)]); });
     }
     if constexpr (FWD_ONLY) {
         fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0, LAZY>(P, a, lds, t, chunk, twf_());
         if constexpr (LO_F != LO_IO) xchg<CB, LO_F, LO_IO>(a, lds, t);
         CFor<0, 16>::run([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            fa_()[base + (yt | ((u32)j << LO_IO))] = a[j];
+            st_stream(&fa_()[base + (yt | ((u32)j << LO_IO))], a[j]);
         });
         return;
     } else {
@@ -328,7 +328,7 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
         if (!SQR && !sqr_rt) {
             if (A.lb && A.bn) load_limbs<16>(P, b, A.lb, A.bn, alpha_(), xs);
             else if (A.lb) CFor<0, 16>::run([&](auto jc) { b[decltype(jc)::value] = 0; });
-            else CFor<0, 16>::run([&](auto jc) { b[decltype(jc)::value] = fb_()[xs(decltype(jc)::value)]; });
+            else CFor<0, 16>::run([&](auto jc) { b[decltype(jc)::value] = ld_stream(&fb_()[xs(decltype(jc)::value)]); });
             if (!A.b_done) {
                 fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0, LAZY>(P, b, lds, t, chunk, twf_());
                 if constexpr (LO_F != 0) xchg<CB, LO_F, 0>(b, lds, t);
@@ -349,7 +349,7 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
         if constexpr (LO_I != LO_IO) xchg<CB, LO_I, LO_IO>(a, lds, t);
         CFor<0, 16>::run([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            fa_()[base + (yt | ((u32)j << LO_IO))] = a[j];
+            st_stream(&fa_()[base + (yt | ((u32)j << LO_IO))], a[j]);
         });
     }
 }
