@@ -218,18 +218,30 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
     // fits in the lower half the partner is zero and the level is a copy.
     // (Uniform per work-group.)
     const bool top_copy = FROM_LIMBS && !INV && blk == 0 && (d << (NLEV - 1)) >= A.an;
+    // Element y of the tile sits at xbase + x0(y), x0(y) = (y & QM) + (y >> Q) d,
+    // and x0 adds over disjoint bit sets: for y = yt | (j << lo) that is a
+    // per-thread base plus an offset that is uniform per register j (one
+    // 64-bit add per access instead of a shift and adds per element).
+    auto x0 = [&](u32 y) -> u64 { return (u64)(y & QM) + ((u64)(y >> Q) << A.g_lo); };
     {
         const u32 yt = lay_t(t, LO_LD);
-        auto xs = [&](int j) {
-            const u32 y = yt | ((u32)j << LO_LD);
-            return xbase + (y & QM) + (u64)(y >> Q) * d;
-        };
+        const u64 xt = xbase + x0(yt);
         if constexpr (FROM_LIMBS) {
+            // (the limb loads keep the per-element index: measured faster
+            // there than the base-plus-offset form)
+            auto xs = [&](int j) {
+                const u32 y = yt | ((u32)j << LO_LD);
+                return xbase + (y & QM) + (u64)(y >> Q) * d;
+            };
             if (A.an == 0) CFor<0, 16>::run([&](auto jc) { v[decltype(jc)::value] = 0; });
             else if (top_copy) load_limbs<8>(P, v, A.limbs, A.an, alpha, xs);
             else load_limbs<16>(P, v, A.limbs, A.an, alpha, xs);
         } else {
-            CFor<0, 16>::run([&](auto jc) { v[decltype(jc)::value] = ld_stream(&data[xs(decltype(jc)::value)]); });
+            const u64* const dt = data + xt;
+            CFor<0, 16>::run([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                v[j] = ld_stream(&dt[x0((u32)j << LO_LD)]);
+            });
         }
     }
     if constexpr (!INV) {
@@ -245,12 +257,10 @@ __device__ __forceinline__ void top_pass_body(const TopArgs& A, u32 pr, u64 tile
     }
     if constexpr (LO_ST != LO_END) xchg<CB, LO_END, LO_ST>(v, lds, t);
     {
-        const u32 yt = lay_t(t, LO_ST);
+        u64* const dt = data + xbase + x0(lay_t(t, LO_ST));
         CFor<0, 16>::run([&](auto jc) {
-            constexpr u32 jy = (u32)decltype(jc)::value << LO_ST;
-            const u32 y = yt | jy;
-            const u64 x = xbase + (y & QM) + (u64)(y >> Q) * d;
-            st_stream(&data[x], v[decltype(jc)::value]);
+            constexpr int j = decltype(jc)::value;
+            st_stream(&dt[x0((u32)j << LO_ST)], v[j]);
         });
     }
 }
@@ -326,8 +336,10 @@ __device__ __forceinline__ void mid_body(const MidArgs& A, u32 pr, u64 chunk, u6
             xchg<CB, LO_IO, 0>(a, lds, t);
         }
         if (!SQR && !sqr_rt) {
-            if (A.lb && A.bn) load_limbs<16>(P, b, A.lb, A.bn, alpha_(), xs);
-            else if (A.lb) CFor<0, 16>::run([&](auto jc) { b[decltype(jc)::value] = 0; });
+            // B comes from limbs only in the kernels that load them
+            // (transforms of one chunk); the others always read fb.
+            if (FROM_LIMBS && A.lb && A.bn) load_limbs<16>(P, b, A.lb, A.bn, alpha_(), xs);
+            else if (FROM_LIMBS && A.lb) CFor<0, 16>::run([&](auto jc) { b[decltype(jc)::value] = 0; });
             else CFor<0, 16>::run([&](auto jc) { b[decltype(jc)::value] = ld_stream(&fb_()[xs(decltype(jc)::value)]); });
             if (!A.b_done) {
                 fwd_rec<CB, CB - 1, 1, CB - 1, LO_IO, 0, LAZY>(P, b, lds, t, chunk, twf_());
