@@ -28,8 +28,15 @@ __host__ __device__ __forceinline__ u64 mk64(u32 lo, u32 hi) { return ((u64)hi <
 // data does not evict the twiddle tables every work-group re-reads from
 // the caches (same-box A/B on the 2^30-bit product: 3.53 -> 3.38 ms).
 // Twiddles and limbs keep ordinary loads.
+// (Plain accesses where the header is compiled by a host compiler without
+// the builtins, as the CPU check of this arithmetic does.)
+#if defined(__clang__)
 __device__ __forceinline__ u64 ld_stream(const u64* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_stream(u64* p, u64 v) { __builtin_nontemporal_store(v, p); }
+#else
+__device__ __forceinline__ u64 ld_stream(const u64* p) { return *p; }
+__device__ __forceinline__ void st_stream(u64* p, u64 v) { *p = v; }
+#endif
 
 // Full 64x64 -> 128-bit product from four 32x32+64 multiply-adds.
 __host__ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi) {
