@@ -18,7 +18,7 @@ HIP_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function"]
 
 SOURCES = ["nnh_mid.hip", "nnh_mid_lazy.hip", "nnh_top.hip", "nnh_top_lazy.hip", "nnh_crt.hip", "nnh_mlp.hip", "nnh_peak.hip",
-           "nnh_nat.hip", "nnh_api.cpp", "nnh_natural.cpp", "nnh_plan.cpp"]
+           "nnh_nat.hip", "nnh_api.cpp", "nnh_natural.cpp", "nnh_plan.cpp", "nnh_hostio.cpp"]
 
 
 def _run(cmd):

@@ -1363,6 +1363,7 @@ int nnh_shutdown(void) {
     g_dev.ws_stream = nullptr;
     drop_twiddles(g_dev);  // batches still alive keep the tables they use
     host_release();
+    hostio_release();
     for (Buf* b : {&g_dev.ws, &g_dev.tiles, &g_dev.stage, &g_dev.tmp, &g_dev.bws, &g_dev.bjobs, &g_dev.btiles}) {
         if (b->p) (void)hipFree(b->p);
         *b = Buf{};
@@ -1442,11 +1443,11 @@ int nnh_mul(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* b, ui
     u64* da = D.stage.p;
     u64* db = da + an;
     u64* dd = db + bn;
-    HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
+    if (int rc_ = host_upload(da, a, an * 8, st)) return rc_;
+    if (int rc_ = host_upload(db, b, bn * 8, st)) return rc_;
     if ((rc = mul_dispatch(D, dd, da, an, db, bn, 1, st))) return rc;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(dst, dd, (an + bn) * 8, hipMemcpyDeviceToHost, st));
+    if (int rc_ = host_download(dst, dd, (an + bn) * 8, st)) return rc_;
     HIPCHK(hipStreamSynchronize(st));
     return NNH_OK;
 }
@@ -1559,11 +1560,11 @@ int nnh_mul_algo(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t* 
     u64* da = D.stage.p;
     u64* db = da + an;
     u64* dd = db + bn;
-    HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
+    if (int rc_ = host_upload(da, a, an * 8, st)) return rc_;
+    if (int rc_ = host_upload(db, b, bn * 8, st)) return rc_;
     if ((rc = mul_algo_device(D, dd, da, an, db, bn, algo, param, st))) return rc;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(dst, dd, (an + bn) * 8, hipMemcpyDeviceToHost, st));
+    if (int rc_ = host_download(dst, dd, (an + bn) * 8, st)) return rc_;
     HIPCHK(hipStreamSynchronize(st));
     return NNH_OK;
 }
@@ -1576,10 +1577,10 @@ int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an) {
     if (rc) return rc;
     u64* da = D.stage.p;
     u64* dd = da + an;
-    HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
+    if (int rc_ = host_upload(da, a, an * 8, st)) return rc_;
     if ((rc = sqr_device(D, dd, da, an, st))) return rc;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(dst, dd, 2 * an * 8, hipMemcpyDeviceToHost, st));
+    if (int rc_ = host_download(dst, dd, 2 * an * 8, st)) return rc_;
     HIPCHK(hipStreamSynchronize(st));
     return NNH_OK;
 }
@@ -1721,10 +1722,10 @@ int nnh_transform_forward(uint64_t* out, const uint64_t* a, uint64_t an, int sca
     int rc;
     if ((rc = ensure_twiddles(D, scale)) || (rc = ensure(D.ws, 6 * pl.L)) || (rc = stage_host(D, an + 16)))
         return rc;
-    HIPCHK(hipMemcpyAsync(D.stage.p, a, an * 8, hipMemcpyHostToDevice, st));
+    if (int rc_ = host_upload(D.stage.p, a, an * 8, st)) return rc_;
     if ((rc = transform_full(D, pl, D.stage.p, an, D.ws.p, st))) return rc;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, D.ws.p, 3 * pl.L * 8, hipMemcpyDeviceToHost, st));
+    if (int rc_ = host_download(out, D.ws.p, 3 * pl.L * 8, st)) return rc_;
     HIPCHK(hipStreamSynchronize(st));
     return NNH_OK;
 }
@@ -1744,11 +1745,11 @@ int nnh_mul_transformed(uint64_t* dst, const uint64_t* fa, uint64_t an, const ui
     u64* fsaved = D.ws.p + 3 * pl.L;  // the reused transform plays operand B
     u64* db = D.stage.p;
     u64* dd = db + bn;
-    HIPCHK(hipMemcpyAsync(fsaved, fa, 3 * pl.L * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
+    if (int rc_ = host_upload(fsaved, fa, 3 * pl.L * 8, st)) return rc_;
+    if (int rc_ = host_upload(db, b, bn * 8, st)) return rc_;
     if ((rc = product_core(D, pl, db, bn, nullptr, an, false, fsaved, dd, an + bn, nullptr, 0, st))) return rc;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(dst, dd, (an + bn) * 8, hipMemcpyDeviceToHost, st));
+    if (int rc_ = host_download(dst, dd, (an + bn) * 8, st)) return rc_;
     HIPCHK(hipStreamSynchronize(st));
     return NNH_OK;
 }
@@ -1770,15 +1771,15 @@ int nnh_mul_cyclic(uint64_t* dst, const uint64_t* a, uint64_t an, const uint64_t
     u64* da = D.stage.p;
     u64* db = da + an;
     u64* dd = db + bn;
-    HIPCHK(hipMemcpyAsync(da, a, an * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(db, b, bn * 8, hipMemcpyHostToDevice, st));
+    if (int rc_ = host_upload(da, a, an * 8, st)) return rc_;
+    if (int rc_ = host_upload(db, b, bn * 8, st)) return rc_;
     if ((rc = product_core(D, pl, da, an, db, bn, false, nullptr, dd, n_out, nullptr, 0, st))) return rc;
     if (cyclic) {
         ProfScope ps(K_FOLD, 16 * L, st);
         launch_cyclic_fold(dd, L, st);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(dst, dd, (cyclic ? L : an + bn) * 8, hipMemcpyDeviceToHost, st));
+    if (int rc_ = host_download(dst, dd, (cyclic ? L : an + bn) * 8, st)) return rc_;
     HIPCHK(hipStreamSynchronize(st));
     return NNH_OK;
 }

@@ -24,6 +24,15 @@ void host_ws_release(hipStream_t st);
 // NNH_ERR_HIP (with its message set) when nnh_debug_fail_next asked for a failure.
 int injected_failure();
 
+// Caller's host buffer -> device (nnh_hostio.cpp): ordered on st; the host
+// buffer has been read completely when the call returns.
+int host_upload(void* d_dst, const void* h_src, u64 bytes, hipStream_t st);
+// Device -> caller's host buffer, after the work queued on st; complete when
+// the call returns.
+int host_download(void* h_dst, const void* d_src, u64 bytes, hipStream_t st);
+// Free the pinned staging slots and stop the copy threads (nnh_shutdown).
+void hostio_release();
+
 // out[0 .. an+bn) = a * b, the reference dispatch (natural.h:444-450).
 int dev_mul(u64* out, const u64* a, u64 an, const u64* b, u64 bn, hipStream_t st);
 // out[0 .. 2 an) = a^2 (natural.h:529-535).

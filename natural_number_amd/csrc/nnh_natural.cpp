@@ -187,14 +187,11 @@ class Ctx {
     }
     DN upload(const u64* h, u64 n) {
         DN x = alloc(n);
-        hipchk(hipMemcpyAsync(x.p, h, n * sizeof(u64), hipMemcpyHostToDevice, st), "upload");
+        rcchk(host_upload(x.p, h, n * sizeof(u64), st));
         sync();
         return x;
     }
-    void download(u64* h, const u64* d, u64 n) {
-        hipchk(hipMemcpyAsync(h, d, n * sizeof(u64), hipMemcpyDeviceToHost, st), "download");
-        sync();
-    }
+    void download(u64* h, const u64* d, u64 n) { rcchk(host_download(h, d, n * sizeof(u64), st)); }
     void copy(u64* dst, const u64* src, u64 n) {
         if (n) hipchk(hipMemcpyAsync(dst, src, n * sizeof(u64), hipMemcpyDeviceToDevice, st), "copy");
     }
@@ -809,8 +806,7 @@ u64 to_dec_into(Ctx& c, DN x, char* out, u64 cap) {
     if (first == total) first = total - 1;  // x == 0 is handled above; keep one digit
     const u64 len = total - first;
     if (len + 1 > cap) raise(NNH_ERR_ARG, "output buffer too small (" + std::to_string(len + 1) + " bytes needed)");
-    hipchk(hipMemcpyAsync(out, dd + first, len, hipMemcpyDeviceToHost, c.st), "digits");
-    c.sync();
+    rcchk(host_download(out, dd + first, len, c.st));
     out[len] = 0;
     tick("copy out");
     return len;
