@@ -41,8 +41,10 @@ typedef enum nnh_status {
  * Reference: the static tables of NTT_info (ntt_info.h:69-145). */
 int nnh_init(int device);
 
-/* Release every device buffer held by the library.  Batches that are still
- * alive keep the twiddle tables they read until nnh_batch_destroy. */
+/* Release every device buffer held by the library, and the pinned host
+ * staging ring and copy threads of the host-pointer calls.  Batches that
+ * are still alive keep the twiddle tables they read until
+ * nnh_batch_destroy. */
 int nnh_shutdown(void);
 
 /* Test hook (no counterpart in the reference): the next n compute calls
