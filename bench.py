@@ -9,6 +9,9 @@ Also reported:
   * the CPU baseline: the oracle (the restatement of the reference CPU path,
     one thread) on the same product, whose result is also compared
     word-for-word with the GPU result;
+  * the same product through the host-buffer call nnh_mul (operands and
+    result in the caller's pageable memory; PCIe-inclusive, never the
+    value);
   * the other plain products at scale (2^30 x 2^22 bits through the
     reference dispatch and the chunked arm; two all-ones 2^28-bit operands
     against the closed form);
@@ -487,6 +490,35 @@ def cost_model_run(n_side, n_held, save_policy=None):
     return {"cost_model": ev}
 
 
+def host_buffer_call(nn, a_h, b_h, reps=4):
+    """The drop-in path: nnh_mul on the caller's pageable host buffers (the
+    reference's naturals live in malloc'd memory), operands copied in and the
+    product copied out by the call.  Wall time per call (best of `reps` after
+    one warm-up), into a reused result buffer and into one whose pages were
+    never touched; PCIe-inclusive, so never the line's value."""
+    import ctypes
+
+    p = ctypes.POINTER(ctypes.c_uint64)
+    L = nn.lib()
+    an, bn = a_h.size, b_h.size
+    out = np.zeros(an + bn, dtype=np.uint64)
+
+    def call(dst):
+        t = time.perf_counter()
+        rc = L.nnh_mul(dst.ctypes.data_as(p), a_h.ctypes.data_as(p), an, b_h.ctypes.data_as(p), bn)
+        if rc:
+            raise RuntimeError("nnh_mul: " + L.nnh_last_error().decode())
+        return time.perf_counter() - t
+
+    log("host-buffer call")
+    call(out)
+    reused = min(call(out) for _ in range(reps))
+    fresh = min(call(np.empty(an + bn, dtype=np.uint64)) for _ in range(2))
+    bits = 64.0 * an
+    return {"ms": round(reused * 1e3, 3), "bits_per_s": bits / reused, "fresh_dst_ms": round(fresh * 1e3, 3),
+            "bytes_moved": 8 * 2 * (an + bn), "result": out}
+
+
 def valu_ceiling(nn, kernels, steps):
     """Butterfly-issue ceiling of the s = 25 product next to the measured kernels.
 
@@ -727,6 +759,7 @@ def main():
     valu = valu_ceiling(nn, kernels, args.steps)
     plans = plans_compare(nn, stream, step, args.steps)
 
+    hb = host_buffer_call(nn, a_h, b_h)
     c2 = c2_variants(nn, stream, max(3, args.steps // 4), not args.no_cpu)
     batch = None if args.no_batch else batch_mix(nn, stream, valu, with_cpu=not args.no_cpu)
     cmodel = None if args.no_policy else policy_eval(nn, args.policy_held_out)
@@ -735,6 +768,7 @@ def main():
 
     res = out.download()
     gpu_digest = hashlib.sha256(res.tobytes()).hexdigest()
+    hb["identical_to_device_path"] = bool(np.array_equal(hb.pop("result"), res))
     cpu = None
     exact = None
     if not args.no_cpu:
@@ -793,6 +827,7 @@ def main():
         "plans": plans,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "host_buffers": hb,
         "c2_variants": c2,
         "batch_mix": batch,
         "cost_model": cmodel,
