@@ -488,9 +488,10 @@ static int join_p2(Device& D, hipStream_t st, hipStream_t st2) {
     return NNH_OK;
 }
 
-// Forward top levels (global bits s-1 .. 12) of operand x into f.
+// Forward top levels (global bits s-1 .. 12) of operand x into f, and of
+// operand y into g when y is set (each pass one launch for both).
 static void fwd_top(const Device& D, const Plan& pl, const u64* x, u64 xn, u64* f, hipStream_t st,
-                    hipStream_t st2) {
+                    hipStream_t st2, const u64* y = nullptr, u64 yn = 0, u64* g = nullptr) {
     bool first = true;
     for (auto [g_lo, nl] : top_passes(pl)) {
         TopPassDesc td{};
@@ -505,9 +506,16 @@ static void fwd_top(const Device& D, const Plan& pl, const u64* x, u64 xn, u64* 
         td.g_lo = (u32)g_lo;
         td.nlev = (u32)nl;
         td.inverse = false;
+        td.two = y != nullptr;
+        if (td.two) {
+            set3(td.data2, g, pl.L);
+            td.limbs2 = first ? y : nullptr;
+            td.an2 = yn;
+        }
         {
             // algorithmic bytes: limbs read once (first pass) or residues read, residues written
             u64 rd = first ? 8 * std::min<u64>(xn, pl.L) : 24 * pl.L;
+            if (td.two) rd += first ? 8 * std::min<u64>(yn, pl.L) + 24 * pl.L : 48 * pl.L;
             ProfScope ps(first ? K_TOP_LOAD : K_TOP_FWD, rd + 24 * pl.L, st);
             launch_top_pass(td, st, st2);
         }
@@ -603,8 +611,8 @@ static int product_core(Device& D, const Plan& pl, const u64* a, u64 an, const u
     int rc = fork_p2(D, st, &st2);
     if (rc) return rc;
     if (!small) {
-        fwd_top(D, pl, a, an, fa, st, st2);
-        if (!sqr && !fb_done) fwd_top(D, pl, b, bn, fb, st, st2);
+        if (!sqr && !fb_done) fwd_top(D, pl, a, an, fa, st, st2, b, bn, fb);
+        else fwd_top(D, pl, a, an, fa, st, st2);
     }
     mid(D, pl, fa, fbp, small ? a : nullptr, an, (small && !sqr && !fb_done) ? b : nullptr, bn, sqr, false,
         false, fb_done != nullptr, st, st2);

@@ -31,6 +31,12 @@ struct TopPassDesc {
     u64 L;
     u32 g_lo, nlev;
     bool inverse;
+    // Optional second operand of a forward pass, transformed by the same
+    // launch (grid z = 2): its arrays, and its limbs when limbs is set.
+    bool two;
+    u64* data2[3];
+    const u64* limbs2;
+    u64 an2;
 };
 
 struct MidDesc {

@@ -9,11 +9,28 @@
 namespace nnh {
 
 // The limb-loading pass asks for 6 waves per SIMD (80 VGPRs; it needs 81
-// unconstrained, which would allow 5).
+// unconstrained, which would allow 5).  A forward pass over both operands of
+// a product runs as one launch: blockIdx.z picks the operand (A, or B), so
+// the two transforms share the GPU and the launch count halves.
+// (The second operand's fields replace the first's one by one: selecting
+// a whole argument struct at run time would put a copy in scratch memory.)
+struct TopOperand {
+    u64* data[3];
+    const u64* limbs;
+    u64 an;
+};
 template <bool INV, bool FROM_LIMBS, int NLEV, bool LAZY, bool TOP>
-__global__ __launch_bounds__(256, FROM_LIMBS ? 6 : 1) void k_top_pass(TopArgs A, u32 pr0) {
+__global__ __launch_bounds__(256, FROM_LIMBS ? 6 : 1) void k_top_pass(TopArgs A, TopOperand B, u32 pr0) {
     __shared__ u32 lds[4096 + 256];  // 32-bit transposes: half the LDS, more work-groups per CU
-    top_pass_body<INV, FROM_LIMBS, NLEV, LAZY, TOP>(A, pr0 + blockIdx.y, blockIdx.x, lds);
+    TopArgs X = A;
+    if (blockIdx.z) {
+        X.data[0] = B.data[0];
+        X.data[1] = B.data[1];
+        X.data[2] = B.data[2];
+        X.limbs = B.limbs;
+        X.an = B.an;
+    }
+    top_pass_body<INV, FROM_LIMBS, NLEV, LAZY, TOP>(X, pr0 + blockIdx.y, blockIdx.x, lds);
 }
 
 // Batched: one work-group per (job, tile) item of a bucket of equal
@@ -49,12 +66,12 @@ __global__ __launch_bounds__(256) void k_top_pass_batch(const BJob* __restrict__
 
 // Host dispatch on the pass depth (1..8 levels), known per launch.
 template <bool INV, bool FL, bool LAZY, bool TOP>
-static void top_single(const TopArgs& A, u32 nlev, dim3 grid, u32 pr0, hipStream_t st) {
+static void top_single(const TopArgs& A, const TopOperand& B, u32 nlev, dim3 grid, u32 pr0, hipStream_t st) {
     switch (nlev) {
 #define NNH_TP(n) \
-    case n: hipLaunchKernelGGL((k_top_pass<INV, FL, n, LAZY, TOP>), grid, dim3(256), 0, st, A, pr0); break;
+    case n: hipLaunchKernelGGL((k_top_pass<INV, FL, n, LAZY, TOP>), grid, dim3(256), 0, st, A, B, pr0); break;
         NNH_TP(1) NNH_TP(2) NNH_TP(3) NNH_TP(4) NNH_TP(5) NNH_TP(6) NNH_TP(7)
-        default: hipLaunchKernelGGL((k_top_pass<INV, FL, 8, LAZY, TOP>), grid, dim3(256), 0, st, A, pr0); break;
+        default: hipLaunchKernelGGL((k_top_pass<INV, FL, 8, LAZY, TOP>), grid, dim3(256), 0, st, A, B, pr0); break;
 #undef NNH_TP
     }
 }
@@ -87,13 +104,19 @@ static void top_pass_primes(const TopPassDesc& D, hipStream_t st) {
     A.an = D.an;
     A.g_lo = D.g_lo;
     A.nlev = D.nlev;
+    TopOperand B{};
+    if (D.two) {
+        for (int i = 0; i < 3; i++) B.data[i] = D.data2[i];
+        B.limbs = D.limbs2;
+        B.an = D.an2;
+    }
     const u32 pr0 = LAZY ? 0 : 2, npr = LAZY ? 2 : 1;
-    dim3 grid((unsigned)(D.L >> 12), npr);
+    dim3 grid((unsigned)(D.L >> 12), npr, D.two ? 2 : 1);
     const bool top = (1ull << (D.g_lo + D.nlev)) == D.L;  // the pass holds the top level
-    if (D.inverse && top) top_single<true, false, LAZY, true>(A, D.nlev, grid, pr0, st);
-    else if (D.inverse) top_single<true, false, LAZY, false>(A, D.nlev, grid, pr0, st);
-    else if (D.limbs) top_single<false, true, LAZY, true>(A, D.nlev, grid, pr0, st);
-    else top_single<false, false, LAZY, false>(A, D.nlev, grid, pr0, st);
+    if (D.inverse && top) top_single<true, false, LAZY, true>(A, B, D.nlev, grid, pr0, st);
+    else if (D.inverse) top_single<true, false, LAZY, false>(A, B, D.nlev, grid, pr0, st);
+    else if (D.limbs) top_single<false, true, LAZY, true>(A, B, D.nlev, grid, pr0, st);
+    else top_single<false, false, LAZY, false>(A, B, D.nlev, grid, pr0, st);
 }
 
 template <bool LAZY>
