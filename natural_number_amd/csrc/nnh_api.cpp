@@ -577,7 +577,7 @@ static void mid(const Device& D, const Plan& pl, u64* fa, const u64* fb, const u
 
 static int crt(Device& D, const Plan& pl, const u64* f, u64 n_res, u64* out, u64 n_out, const u64* addend,
                u64 n_add, hipStream_t st) {
-    u64 tiles = crt_tiles(n_out);
+    u64 tiles = crt_tiles_single(n_out);
     int rc = ensure(D.tiles, (2 * tiles + 7) / 8 + 16);
     if (rc) return rc;
     CrtDesc cd{};
@@ -662,7 +662,7 @@ static int school(Device& D, const u64* a, u64 an, const u64* b, u64 bn, u64* ou
         launch_school_cols(a, an, b, bn, c0, c1, c2, st);
     }
     u64 n_out = an + bn;
-    u64 tiles = crt_tiles(n_out);
+    u64 tiles = crt_tiles_single(n_out);
     rc = ensure(D.tiles, (2 * tiles + 7) / 8 + 16);
     if (rc) return rc;
     CrtDesc cd{};
@@ -730,7 +730,7 @@ static int kara_device(Device& D, u64* out, const u64* a, u64 an, const u64* b, 
         launch_kara(a, an, b, bn, D.tmp.p, cols, st);
     }
     u64 n_out = an + bn;
-    u64 tiles = crt_tiles(n_out);
+    u64 tiles = crt_tiles_single(n_out);
     if ((rc = ensure(D.tiles, (2 * tiles + 7) / 8 + 16))) return rc;
     CrtDesc cd{};
     cd.r[0] = cols; cd.r[1] = cols + nc; cd.r[2] = cols + 2 * nc;

@@ -56,10 +56,11 @@ __device__ __forceinline__ void crt_x(const CrtArgs& A, u64 k, u64& x0, u64& x1,
     crt_finish(A, k, crt_load(A, k), x0, x1, x2);
 }
 
+template <int ROUNDS = CRT_TILE_ROUNDS>
 __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn_slot, u64 (*xs)[256 + 3],
                                               u32 (*wco)[4]) {
     const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const u64 tile_base = tile * (256 * CRT_TILE_ROUNDS);
+    const u64 tile_base = tile * (256 * ROUNDS);
     __syncthreads();  // LDS reuse across persistent iterations
     if (t < 3) {
         u64 x0 = 0, x1 = 0, x2 = 0;
@@ -70,11 +71,11 @@ __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn
     // each round's words are loaded one round ahead, so that their HBM
     // latency overlaps the previous round's reconstruction and carries
     CrtIn next = crt_load(A, tile_base + t);
-    for (int r = 0; r < CRT_TILE_ROUNDS; r++) {
+    for (int r = 0; r < ROUNDS; r++) {
         const u64 k = tile_base + (u64)r * 256 + t;
         if (k >= A.n_out && tile_base + (u64)r * 256 >= A.n_out) break;  // uniform
         const CrtIn cur = next;
-        if (r + 1 < CRT_TILE_ROUNDS) next = crt_load(A, k + 256);
+        if (r + 1 < ROUNDS) next = crt_load(A, k + 256);
         u64 x0, x1, x2;
         crt_finish(A, k, cur, x0, x1, x2);
         __syncthreads();
@@ -176,10 +177,11 @@ __device__ __forceinline__ void carry_scan_body(const u8* tile_fn, u64 n_tiles, 
 
 // Add the tile carry-in (0/1) at the tile's first word and propagate it
 // through the tile's all-ones words.
+template <int ROUNDS = CRT_TILE_ROUNDS>
 __device__ __forceinline__ void carry_fix_body(u64* out, u64 n_out, u64 tile, u32& first_nz) {
     const u32 t = threadIdx.x;
-    const u64 tile_base = tile * (256 * CRT_TILE_ROUNDS);
-    for (int r = 0; r < CRT_TILE_ROUNDS; r++) {
+    const u64 tile_base = tile * (256 * ROUNDS);
+    for (int r = 0; r < ROUNDS; r++) {
         u64 base = tile_base + (u64)r * 256;
         if (base >= n_out) return;
         u64 k = base + t;

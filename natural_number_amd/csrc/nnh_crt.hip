@@ -22,10 +22,11 @@ __global__ void k_twiddles(u64* tw, u64 n, RootBases rb, int inv, int pr) {
     tw[t] = acc;
 }
 
+template <int ROUNDS>
 __global__ __launch_bounds__(256) void k_crt_tile(CrtArgs A) {
     __shared__ u64 xs[3][256 + 3];
     __shared__ u32 wco[2][4];
-    crt_tile_body(A, blockIdx.x, A.tile_fn + blockIdx.x, xs, wco);
+    crt_tile_body<ROUNDS>(A, blockIdx.x, A.tile_fn + blockIdx.x, xs, wco);
 }
 
 __global__ __launch_bounds__(1024) void k_carry_scan(u8* tile_fn, u64 n_tiles, u8* tile_cin) {
@@ -33,10 +34,11 @@ __global__ __launch_bounds__(1024) void k_carry_scan(u8* tile_fn, u64 n_tiles, u
     carry_scan_body(tile_fn, n_tiles, tile_cin, chunk_fn);
 }
 
+template <int ROUNDS>
 __global__ __launch_bounds__(256) void k_carry_fix(u64* out, u64 n_out, const u8* tile_cin) {
     __shared__ u32 first_nz;
     if (!tile_cin[blockIdx.x]) return;
-    carry_fix_body(out, n_out, blockIdx.x, first_nz);
+    carry_fix_body<ROUNDS>(out, n_out, blockIdx.x, first_nz);
 }
 
 __device__ __forceinline__ u32 find_job_by_tile(const BJob* jobs, u32 njobs, u64 tile) {
@@ -302,12 +304,19 @@ void launch_crt(const CrtDesc& D, hipStream_t st) {
     A.n_add = D.n_add;
     A.tile_fn = D.tile_fn;
     A.C = D.C;
-    u64 tiles = (D.n_out + 256 * CRT_TILE_ROUNDS - 1) / (256 * CRT_TILE_ROUNDS);
-    hipLaunchKernelGGL(k_crt_tile, dim3((unsigned)tiles), dim3(256), 0, st, A);
+    const u64 tiles = crt_tiles_single(D.n_out);
+    const bool sh = crt_tile_words_single(D.n_out) == CRT_SHORT_TILE_WORDS;
+    static_assert(CRT_SHORT_TILE_WORDS == 2 * 256, "short CRT tiles are two rounds");
+    if (sh) hipLaunchKernelGGL(k_crt_tile<2>, dim3((unsigned)tiles), dim3(256), 0, st, A);
+    else hipLaunchKernelGGL(k_crt_tile<CRT_TILE_ROUNDS>, dim3((unsigned)tiles), dim3(256), 0, st, A);
     if (tiles > 1) {
         hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, st, D.tile_fn, tiles, D.tile_cin);
-        hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)tiles), dim3(256), 0, st, D.out, D.n_out,
-                           (const u8*)D.tile_cin);
+        if (sh)
+            hipLaunchKernelGGL(k_carry_fix<2>, dim3((unsigned)tiles), dim3(256), 0, st, D.out, D.n_out,
+                               (const u8*)D.tile_cin);
+        else
+            hipLaunchKernelGGL(k_carry_fix<CRT_TILE_ROUNDS>, dim3((unsigned)tiles), dim3(256), 0, st, D.out,
+                               D.n_out, (const u8*)D.tile_cin);
     }
 }
 

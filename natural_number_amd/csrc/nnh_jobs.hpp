@@ -13,6 +13,17 @@ typedef uint8_t u8;
 // Words per CRT / carry-lookahead tile (16 rounds of 256 positions).
 constexpr u64 CRT_TILE_WORDS = 4096;
 inline u64 crt_tiles(u64 n_out) { return (n_out + CRT_TILE_WORDS - 1) / CRT_TILE_WORDS; }
+// A single product's CRT: outputs shorter than 2^22 words use tiles of 512
+// words (two rounds), so that enough work-groups run side by side (a
+// 2^17-word output is 32 tiles of 4096 words, each 16 rounds in sequence);
+// batches keep 4096-word tiles.
+constexpr u64 CRT_SHORT_TILE_WORDS = 512;
+constexpr u64 CRT_SHORT_BELOW = 1ull << 22;
+inline u64 crt_tile_words_single(u64 n_out) { return n_out < CRT_SHORT_BELOW ? CRT_SHORT_TILE_WORDS : CRT_TILE_WORDS; }
+inline u64 crt_tiles_single(u64 n_out) {
+    const u64 w = crt_tile_words_single(n_out);
+    return (n_out + w - 1) / w;
+}
 
 // One product of a batch (device-resident table).
 struct BJob {

@@ -104,6 +104,33 @@ def test_all_ones_square_closed_form(nnh):
     assert np.array_equal(nnh.sqr(a), exp)
 
 
+@pytest.mark.parametrize("n", [1000, 40000, (1 << 20) + 3, (1 << 21) - 1])
+def test_all_ones_square_short_crt_tiles(nnh, n):
+    # Outputs below 2^22 words reconstruct in 512-word CRT tiles; the upper
+    # half of (B^n - 1)^2 is all ones, so a carry entering it crosses every
+    # tile there.  Square via mul and sqr.
+    a = np.full(n, 2**64 - 1, dtype=np.uint64)
+    say(f"all-ones {n}-word square")
+    exp = np.empty(2 * n, dtype=np.uint64)
+    exp[0] = 1
+    exp[1:n] = 0
+    exp[n] = 2**64 - 2
+    exp[n + 1:] = 2**64 - 1
+    assert np.array_equal(nnh.mul(a, a.copy()), exp)
+    assert np.array_equal(nnh.sqr(a), exp)
+
+
+@pytest.mark.parametrize("n,m", [(300000, 70000), (70001, 1 << 19)])
+def test_all_ones_unbalanced_closed_form(nnh, n, m):
+    # (B^n - 1)(B^m - 1) = B^(n+m) - B^n - B^m + 1
+    a = np.full(n, 2**64 - 1, dtype=np.uint64)
+    b = np.full(m, 2**64 - 1, dtype=np.uint64)
+    say(f"all-ones {n}x{m}")
+    B = 64
+    want = (1 << (B * (n + m))) - (1 << (B * n)) - (1 << (B * m)) + 1
+    assert to_int(nnh.mul(a, b)) == want
+
+
 @pytest.mark.parametrize("scale,n", [(8, 100), (10, 600), (12, 4096), (13, 5000), (16, 40000), (21, 1 << 20)])
 def test_forward_transform_stage(nnh, oracle, scale, n):
     """Device transform == alpha * reference transform (mod p), every word."""
