@@ -75,12 +75,22 @@ int hip_rc(hipError_t e, const char* what) {
 
 }  // namespace
 
+// The runtime's own pageable copies, for small transfers and when the ring
+// cannot be set up (no pinned memory left): slower, never wrong.
+static int plain_upload(void* d_dst, const void* h_src, u64 bytes, hipStream_t st) {
+    return hip_rc(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, st), "upload");
+}
+static int plain_download(void* h_dst, const void* d_src, u64 bytes, hipStream_t st) {
+    if (int rc = hip_rc(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, st), "download")) return rc;
+    return hip_rc(hipStreamSynchronize(st), "download");
+}
+
 int host_upload(void* d_dst, const void* h_src, u64 bytes, hipStream_t st) {
     if (!bytes) return NNH_OK;
-    if (bytes < STAGED_MIN) return hip_rc(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, st), "upload");
+    if (bytes < STAGED_MIN) return plain_upload(d_dst, h_src, bytes, st);
     std::lock_guard<std::mutex> g(g_ring_mu);
     Ring& R = g_ring;
-    if (int rc = ring_ready(R)) return rc;
+    if (ring_ready(R) != NNH_OK) return plain_upload(d_dst, h_src, bytes, st);
     const u64 nchunk = (bytes + CHUNK - 1) / CHUNK;
     for (u64 c = 0; c < nchunk; c++) {
         const int i = (int)(c % NSLOT);
@@ -97,13 +107,10 @@ int host_upload(void* d_dst, const void* h_src, u64 bytes, hipStream_t st) {
 
 int host_download(void* h_dst, const void* d_src, u64 bytes, hipStream_t st) {
     if (!bytes) return NNH_OK;
-    if (bytes < STAGED_MIN) {
-        if (int rc = hip_rc(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, st), "download")) return rc;
-        return hip_rc(hipStreamSynchronize(st), "download");
-    }
+    if (bytes < STAGED_MIN) return plain_download(h_dst, d_src, bytes, st);
     std::lock_guard<std::mutex> g(g_ring_mu);
     Ring& R = g_ring;
-    if (int rc = ring_ready(R)) return rc;
+    if (ring_ready(R) != NNH_OK) return plain_download(h_dst, d_src, bytes, st);
     const u64 nchunk = (bytes + CHUNK - 1) / CHUNK;
     auto issue = [&](u64 c) -> int {
         const int i = (int)(c % NSLOT);
