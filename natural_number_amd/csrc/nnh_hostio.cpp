@@ -8,7 +8,7 @@
 // 30 GB/s host-to-device and 17.6 GB/s device-to-host on MI355X (the 2^30-bit
 // product's 512 MiB of operands and result take 24 of its 28 ms).  Large
 // transfers here go through a ring of pinned slots instead: the host copies
-// chunk c+1 between the caller's buffer and a slot (split over a few
+// chunk c+1 between the caller's buffer and a slot (split over six
 // threads) while the DMA engine moves chunk c, each slot reused once the
 // event recorded after its last DMA has completed.  Pinned DMA runs at
 // 57 GB/s each way.
@@ -35,7 +35,7 @@ struct Ring {
     void* slot[NSLOT]{};
     hipEvent_t ev[NSLOT]{};
     bool busy[NSLOT]{};  // an event was recorded after the slot's last DMA
-    CopyTeam<3> team;  // three copy threads beside the caller
+    CopyTeam<5> team;  // five copy threads beside the caller (faster into untouched result pages than three)
 };
 Ring g_ring;
 std::mutex g_ring_mu;
