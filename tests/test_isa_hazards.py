@@ -1,0 +1,163 @@
+"""SGPR wait states of the hand-written carry chains, checked in the gfx950 code.
+
+The Montgomery products in natural_number_amd/csrc/nnh_arith.hpp are inline
+assembly that carries 64-bit column sums through SGPR carry masks
+(v_mad_u64_u32 / v_add_co / v_addc / v_subb with an SGPR carry-out read by a
+later instruction).  On gfx950 a VALU instruction that reads an SGPR written
+by another VALU instruction needs two wait states between them; the compiler
+inserts them (s_nop) for the code it generates, but not inside inline
+assembly, whose blocks the source orders and pads by hand.
+
+This test compiles every translation unit that contains those blocks to
+gfx950 assembly (device code only, the library's flags) and scans each
+straight-line run of instructions: every VALU read of an SGPR (or VCC) that a
+VALU instruction wrote must come at least two wait states after the write
+(each instruction counts one, s_nop N counts N + 1).  It fails if a compiler
+or scheduling change ever places the blocks so that a carry is read too early.
+A sensitivity check asks the same scan for three wait states and expects it
+to find the carry reads of the inline blocks that sit exactly at two, so that
+a scanner which parsed nothing cannot pass.  No GPU is needed.
+"""
+import os
+import re
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "natural_number_amd", "csrc")
+UNITS = ["nnh_mid_lazy.hip", "nnh_mid.hip", "nnh_top_lazy.hip", "nnh_top.hip", "nnh_crt.hip", "nnh_peak.hip"]
+FLAGS = ["-O3", "-std=c++20", "--offload-arch=gfx950", "--cuda-device-only", "-S",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+# VOP3 forms whose second operand is an SGPR destination (carry-out)
+_SDST = re.compile(r"^v_(add|sub|subrev|addc|subb|subbrev)_co_u32_e64$|^v_mad_[ui]64_[ui]32$|^v_div_scale")
+_BRANCH = ("s_cbranch", "s_branch", "s_setpc_b64", "s_endpgm")
+
+
+def _sgprs(tok):
+    tok = tok.strip()
+    m = re.match(r"^s\[(\d+):(\d+)\]$", tok)
+    if m:
+        return set(range(int(m.group(1)), int(m.group(2)) + 1))
+    m = re.match(r"^s(\d+)$", tok)
+    if m:
+        return {int(m.group(1))}
+    if tok == "vcc":
+        return {"vcc_lo", "vcc_hi"}
+    if tok in ("vcc_lo", "vcc_hi"):
+        return {tok}
+    return set()
+
+
+def scan(text, need=2):
+    """Return (violations, inline_reads_at): each violation is (function,
+    line, wait states, instruction); inline_reads_at counts VALU SGPR reads
+    inside inline-asm blocks by their wait-state distance from the write."""
+    viol, dist = [], {}
+    func, last_w, pos, in_asm = None, {}, 0, False
+    for ln, raw in enumerate(text.splitlines(), 1):
+        st = raw.strip()
+        if st.startswith(";;#ASMSTART"):
+            in_asm = True
+            continue
+        if st.startswith(";;#ASMEND"):
+            in_asm = False
+            continue
+        s = raw.split(";")[0].strip()
+        if not s:
+            continue
+        if s.endswith(":"):
+            # a function starts afresh; a block label keeps the state of the
+            # code that falls through to it (a branch resets it, below)
+            if not s.startswith("."):
+                func, last_w = s[:-1], {}
+            continue
+        if s.startswith("."):
+            continue
+        parts = s.split(None, 1)
+        mn = parts[0]
+        ops = [o.strip() for o in parts[1].split(",")] if len(parts) > 1 else []
+        if mn == "s_nop":
+            pos += int(ops[0], 0) + 1
+            continue
+        pos += 1
+        if mn.startswith(_BRANCH):
+            last_w = {}
+            continue
+        if not mn.startswith("v_"):
+            if ops:  # a scalar write replaces the VALU value of that register
+                for r in _sgprs(ops[0]):
+                    last_w.pop(r, None)
+            continue
+        dst, src = set(), ops[1:]
+        if mn.startswith("v_cmp"):
+            dst = _sgprs(ops[0]) if ops else set()
+        elif _SDST.match(mn):
+            dst, src = _sgprs(ops[1]), ops[2:]
+        elif mn.endswith("_e32") and "_co_" in mn and len(ops) > 1 and ops[1] == "vcc":
+            dst, src = _sgprs("vcc"), ops[2:]
+        elif mn.startswith(("v_readfirstlane", "v_readlane")):
+            dst = _sgprs(ops[0])
+        if mn.endswith("_e32") and mn.startswith(("v_cndmask", "v_addc", "v_subb")):
+            src = src + ["vcc"]
+        near = [pos - last_w[r] - 1 for o in src for r in _sgprs(o) if r in last_w]
+        if near:
+            ws = min(near)
+            if in_asm:
+                dist[ws] = dist.get(ws, 0) + 1
+            if ws < need:
+                viol.append((func, ln, ws, s))
+        for r in dst:
+            last_w[r] = pos
+    return viol, dist
+
+
+def _compile(src, out_dir):
+    out = os.path.join(out_dir, os.path.splitext(src)[0] + ".s")
+    r = subprocess.run(["hipcc", src, "-o", out] + FLAGS, cwd=CSRC, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return src, open(out).read()
+
+
+@pytest.fixture(scope="module")
+def gfx950_asm(tmp_path_factory):
+    out_dir = str(tmp_path_factory.mktemp("isa"))
+    jobs = max(1, min(len(UNITS), os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as ex:
+        return dict(ex.map(lambda u: _compile(u, out_dir), UNITS))
+
+
+def test_inline_carry_chains_keep_two_wait_states(gfx950_asm):
+    bad = {}
+    for unit, text in gfx950_asm.items():
+        assert ";;#ASMSTART" in text, f"{unit}: no inline assembly found"
+        viol, _ = scan(text, need=2)
+        if viol:
+            bad[unit] = viol[:10]
+    assert not bad, f"VALU SGPR reads closer than two wait states to their write: {bad}"
+
+
+def test_scan_sees_the_inline_carries(gfx950_asm):
+    # the carry chains are padded to exactly two wait states in places: a
+    # scan that asks for three must report those reads
+    text = gfx950_asm["nnh_mid_lazy.hip"]
+    viol, dist = scan(text, need=3)
+    assert dist.get(2, 0) > 0, dist
+    assert viol
+
+
+def test_scan_flags_a_planted_early_read():
+    # a carry read right after its write, as a block without its s_nop would be
+    text = "\n".join([
+        "k:",
+        ";;#ASMSTART",
+        "v_add_co_u32_e64 v1, s[4:5], v2, v3",
+        "v_addc_co_u32_e64 v6, s[8:9], v7, 0, s[4:5]",
+        ";;#ASMEND",
+    ])
+    viol, _ = scan(text, need=2)
+    assert len(viol) == 1 and viol[0][2] == 0
+    padded = text.replace("v_addc", "s_nop 1\nv_addc")
+    assert scan(padded, need=2)[0] == []
