@@ -262,11 +262,9 @@ static void build_constants(Device& D) {
     }
     u64 m0 = P[0], m1 = P[1], m2 = P[2];
     D.crt.c02m = to_mont(powmod(m0, m2 - 2, m2), m2);
-    D.crt.m0m = to_mont(m0 % m1, m1);
-    D.crt.c1m = to_mont(powmod(mulmod(m0 % m1, m2 % m1, m1), m1 - 2, m1), m1);
-    u128 m02 = (u128)m0 * m2;
-    D.crt.m02lo = (u64)m02;
-    D.crt.m02hi = (u64)(m02 >> 64);
+    const u64 c1 = powmod(mulmod(m0 % m1, m2 % m1, m1), m1 - 2, m1);
+    D.crt.c1m = to_mont(c1, m1);
+    D.crt.km = to_mont((m1 - mulmod(m0 % m1, c1, m1)) % m1, m1);
 }
 
 static int ensure_twiddles(Device& D, int s) {

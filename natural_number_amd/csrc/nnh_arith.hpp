@@ -360,24 +360,37 @@ __host__ __device__ __forceinline__ void base_mul4(const Prime& P, u64& x0, u64&
 // Garner reconstruction of X < p0*p1*p2 < 2^186 from canonical residues, in
 // the order of the reference's asmCRT (asmLinux.asm:1369-1450): first p0 and
 // p2, then p1.  Constants: c02m = (p0^-1 mod p2)*2^64 mod p2,
-// m0m = (p0 mod p1)*2^64 mod p1, c1m = ((p0 p2)^-1 mod p1)*2^64 mod p1,
-// m02 = p0*p2 (two words).  Output X as three words x0 + x1 2^64 + x2 2^128.
-__host__ __device__ __forceinline__ void garner(u64 r0, u64 r1, u64 r2, u64 c02m, u64 m0m, u64 c1m, u64 m02lo,
-                                                u64 m02hi, u64& x0, u64& x1, u64& x2) {
+// c1m = ((p0 p2)^-1 mod p1)*2^64 mod p1, km = (-p0 (p0 p2)^-1 mod p1)*2^64
+// mod p1.  Output X as three words x0 + x1 2^64 + x2 2^128.
+//   u = (r2 - r0) / p0 mod p2,  x02 = r0 + p0 u,
+//   v = (r1 - x02) / (p0 p2) mod p1 = (r1 - r0) c1 + (u mod p1) k  (mod p1),
+//   X = x02 + p0 p2 v,
+// the last with p0 p2 = k0 k2 2^112 + (k0 + k2) 2^56 + 1 known at compile
+// time, and v as one double Montgomery product (one reduction for both
+// terms) instead of two products and a subtraction.
+__host__ __device__ __forceinline__ void garner(u64 r0, u64 r1, u64 r2, u64 c02m, u64 c1m, u64 km, u64& x0,
+                                                u64& x1, u64& x2) {
     constexpr Prime Q0 = prime_of(0), Q1 = prime_of(1), Q2 = prime_of(2);
     constexpr u64 P0 = Q0.p, P1 = Q1.p, P2 = Q2.p;
+    static_assert(PRIME_K[0] + PRIME_K[2] < 256, "p0 p2 middle term fits one byte");
+    constexpr u64 M02LO = ((u64)(PRIME_K[0] + PRIME_K[2]) << 56) + 1;  // p0 p2 mod 2^64
+    constexpr u64 M02HI = (u64)(PRIME_K[0] * PRIME_K[2]) << 48;         // p0 p2 >> 64
     u64 d = r2 + P2 - r0;                                  // (0, 2 p2)
     u64 u = csub(Q2, mont(Q2, d, c02m));                   // (r2 - r0)/p0 mod p2
     u64 lo, hi;
     mul_wide(u, P0, lo, hi);
     lo += r0;
     hi += lo < r0 ? 1u : 0u;                               // x02 = r0 + p0 u
-    u64 tm = csub(Q1, mont(Q1, u, m0m));                   // p0 u mod p1
-    u64 e = r1 + (P1 - r0) + (P1 - tm);                    // r1 - x02 (mod p1), < 3 p1
-    u64 v = csub(Q1, mont(Q1, e, c1m));                    // / (p0 p2) mod p1
+    const u64 e = csub(Q1, r1 + P1 - r0);                  // r1 - r0 mod p1
+    const u64 uu = csub(Q1, u);                            // u mod p1 (u < p2 < 2 p1)
+#if defined(__HIP_DEVICE_COMPILE__)
+    const u64 v = csub(Q1, mont2_dev(Q1, e, c1m, uu, km));
+#else
+    const u64 v = csub(Q1, mont2(Q1, e, c1m, uu, km));
+#endif
     u64 l1, h1, l2, h2;
-    mul_wide(v, m02lo, l1, h1);
-    mul_wide(v, m02hi, l2, h2);
+    mul_wide(v, M02LO, l1, h1);
+    mul_wide(v, M02HI, l2, h2);
     x0 = lo + l1;
     u64 mid = hi + h1 + (x0 < lo ? 1u : 0u);
     x1 = mid + l2;

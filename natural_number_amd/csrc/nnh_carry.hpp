@@ -48,8 +48,7 @@ __device__ __forceinline__ void crt_finish(const CrtArgs& A, u64 k, const CrtIn&
     else {
         // p0 and p1 leave the inverse transform in [0, 2p) (lazy butterflies)
         constexpr Prime Q0 = prime_of(0), Q1 = prime_of(1);
-        garner(csub(Q0, in.r0), csub(Q1, in.r1), in.r2, A.C.c02m, A.C.m0m, A.C.c1m, A.C.m02lo, A.C.m02hi, x0, x1,
-               x2);
+        garner(csub(Q0, in.r0), csub(Q1, in.r1), in.r2, A.C.c02m, A.C.c1m, A.C.km, x0, x1, x2);
     }
 }
 __device__ __forceinline__ void crt_x(const CrtArgs& A, u64 k, u64& x0, u64& x1, u64& x2) {

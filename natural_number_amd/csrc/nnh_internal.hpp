@@ -17,9 +17,8 @@ struct RootBases {
 // Garner constants (Montgomery form where they multiply residues).
 struct CrtConsts {
     u64 c02m;   // m0^-1 mod m2, times 2^64 mod m2
-    u64 m0m;    // m0 mod m1, times 2^64 mod m1
     u64 c1m;    // (m0 m2)^-1 mod m1, times 2^64 mod m1
-    u64 m02lo, m02hi;  // m0 * m2
+    u64 km;     // -m0 (m0 m2)^-1 mod m1, times 2^64 mod m1
 };
 
 struct TopPassDesc {

@@ -110,9 +110,9 @@ static int check_prime(std::mt19937_64& rng, int iters, int pi) {
 static int check_garner(std::mt19937_64& rng, int iters) {
     const u64 m0 = prime_of(0).p, m1 = prime_of(1).p, m2 = prime_of(2).p;
     u64 c02m = to_mont(powmod(m0, m2 - 2, m2), m2);
-    u64 m0m = to_mont(m0 % m1, m1);
-    u64 c1m = to_mont(powmod((u64)((u128)(m0 % m1) * (m2 % m1) % m1), m1 - 2, m1), m1);
-    u128 m02 = (u128)m0 * m2;
+    u64 c1 = powmod((u64)((u128)(m0 % m1) * (m2 % m1) % m1), m1 - 2, m1);
+    u64 c1m = to_mont(c1, m1);
+    u64 km = to_mont((m1 - (u64)((u128)(m0 % m1) * c1 % m1)) % m1, m1);
     int bad = 0;
     for (int it = 0; it < iters; it++) {
         // random X < m0 m1 m2 built from random residues, or extreme residues
@@ -121,7 +121,7 @@ static int check_garner(std::mt19937_64& rng, int iters) {
         if (it % 8 == 2) { r0 = 0; r1 = 0; r2 = 0; }
         if (it % 8 == 3) { r0 = m0 - 1; r1 = 0; r2 = m2 - 1; }
         u64 x0, x1, x2;
-        garner(r0, r1, r2, c02m, m0m, c1m, (u64)m02, (u64)(m02 >> 64), x0, x1, x2);
+        garner(r0, r1, r2, c02m, c1m, km, x0, x1, x2);
         // check X mod each prime via the 3-word value
         auto modp = [&](u64 p) {
             u128 t = ((u128)(x2 % p) << 64 | x1) % p;
