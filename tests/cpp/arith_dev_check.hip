@@ -1,9 +1,9 @@
 // Device check of the modular arithmetic in nnh_arith.hpp as the kernels
 // compile it (the asm Montgomery products, the fused-addend lazy forward
-// butterfly, the twiddle-1 butterflies, the asm base multiply): each
-// primitive runs on the GPU over random and boundary inputs, and the host
-// checks every output's range and its value mod p against exact 128-bit
-// arithmetic.  Built by the package build next to this file; run by
+// butterfly, the twiddle-1 butterflies, the asm base multiply, Garner's
+// reconstruction): each primitive runs on the GPU over random and boundary
+// inputs, and the host checks every output's range and its value mod p
+// against exact 128-bit arithmetic.  Built by the package build next to this file; run by
 // tests/test_gpu_arith.py.  Exit status 0 = every check passes.
 #include <hip/hip_runtime.h>
 
@@ -86,6 +86,68 @@ static u64 powmod(u64 b, u64 e, u64 p) {
     for (b %= p; e; e >>= 1, b = (u64)((u128)b * b % p))
         if (e & 1) r = (u64)((u128)r * b % p);
     return r;
+}
+
+// Garner reconstruction as the CRT kernels compile it: canonical residues
+// (r0, r1, r2) per case in, X as three words out.
+__global__ void k_garner(const u64* __restrict__ in, u64* __restrict__ out, u64 c02m, u64 c1m, u64 km, u32 n) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    garner(in[3 * (u64)i], in[3 * (u64)i + 1], in[3 * (u64)i + 2], c02m, c1m, km, out[3 * (u64)i],
+           out[3 * (u64)i + 1], out[3 * (u64)i + 2]);
+}
+
+static u64 to_mont(u64 x, u64 p) { return (u64)(((u128)x << 64) % p); }
+
+// X < p0 p1 p2 with X mod p_i = r_i for random and boundary residues.
+static int check_garner() {
+    const u64 m0 = prime_of(0).p, m1 = prime_of(1).p, m2 = prime_of(2).p;
+    const u64 c02m = to_mont(powmod(m0, m2 - 2, m2), m2);
+    const u64 c1 = powmod((u64)((u128)(m0 % m1) * (m2 % m1) % m1), m1 - 2, m1);
+    const u64 c1m = to_mont(c1, m1);
+    const u64 km = to_mont((m1 - (u64)((u128)(m0 % m1) * c1 % m1)) % m1, m1);
+    const u32 n = 1u << 18;
+    std::vector<u64> in(3 * (size_t)n), out(3 * (size_t)n);
+    std::mt19937_64 rng(4321);
+    const u64 ms[3] = {m0, m1, m2};
+    for (u32 i = 0; i < n; i++)
+        for (int j = 0; j < 3; j++) {
+            const u64 p = ms[j];
+            const u64 edge[] = {0, 1, 2, p - 1, p - 2, p / 2, (p - 1) & ~0xffffffffull, 0xffffffffull};
+            in[3 * (size_t)i + j] = i < n / 4 ? edge[rng() % 8] : rng() % p;
+        }
+    u64 *din = nullptr, *dout = nullptr;
+    if (hipMalloc(&din, in.size() * 8) != hipSuccess || hipMalloc(&dout, out.size() * 8) != hipSuccess ||
+        hipMemcpy(din, in.data(), in.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        std::printf("garner: device set-up failed\n");
+        return 1;
+    }
+    hipLaunchKernelGGL(k_garner, dim3(n / 256), dim3(256), 0, 0, din, dout, c02m, c1m, km, n);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out.data(), dout, out.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+        std::printf("garner: kernel failed\n");
+        return 1;
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    // M = p0 p1 p2 as three words, for the range check
+    const u128 m01 = (u128)m0 * m1;
+    const u128 lo = (u128)(u64)m01 * m2, hi = (u128)(u64)(m01 >> 64) * m2 + (lo >> 64);
+    const u64 M[3] = {(u64)lo, (u64)hi, (u64)(hi >> 64)};
+    long bad = 0;
+    for (u32 i = 0; i < n; i++) {
+        const u64* x = &out[3 * (size_t)i];
+        bool ok = true;
+        for (int j = 0; j < 3; j++) {
+            const u64 p = ms[j];
+            const u128 t = ((u128)(x[2] % p) << 64 | x[1]) % p;
+            ok = ok && (u64)(((t << 64) | x[0]) % p) == in[3 * (size_t)i + j];
+        }
+        const bool below = x[2] != M[2] ? x[2] < M[2] : (x[1] != M[1] ? x[1] < M[1] : x[0] < M[0]);
+        if (!ok || !below) bad++;
+    }
+    std::printf("garner: %u cases; %ld failing\n", n, bad);
+    return bad ? 1 : 0;
 }
 
 int main() {
@@ -184,5 +246,6 @@ int main() {
         std::printf(bp ? "\n" : " none\n");
         bad_total += bp;
     }
+    bad_total += check_garner();
     return bad_total ? 1 : 0;
 }

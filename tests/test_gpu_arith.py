@@ -1,7 +1,8 @@
 """The modular arithmetic as the kernels compile it (asm Montgomery products,
 the fused-addend lazy forward butterfly, twiddle-1 butterflies, the asm base
-multiply), run on the GPU over random and boundary inputs and checked on the
-host against exact 128-bit arithmetic (tests/cpp/arith_dev_check.hip)."""
+multiply, Garner's reconstruction), run on the GPU over random and boundary
+inputs and checked on the host against exact 128-bit arithmetic
+(tests/cpp/arith_dev_check.hip)."""
 import os
 import subprocess
 
