@@ -55,10 +55,75 @@ __device__ __forceinline__ void crt_x(const CrtArgs& A, u64 k, u64& x0, u64& x1,
     crt_finish(A, k, crt_load(A, k), x0, x1, x2);
 }
 
+// One round of a tile: positions k = tile_base + 256 r + t.  FULL: every
+// position of the tile carries residues and an output word and there is no
+// addend, so the round needs no range checks (the common case of a product's
+// tiles; the last tile and column or addend jobs take the general form).
+template <bool FULL>
+__device__ __forceinline__ void crt_round(const CrtArgs& A, u64 k, const CrtIn& cur, u64 (*xs)[256 + 3],
+                                          u32 (*wco)[4], u32& cin0, u32& cin1) {
+    const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    u64 x0, x1, x2;
+    if constexpr (FULL) {
+        // p0 and p1 leave the inverse transform in [0, 2p) (lazy butterflies)
+        constexpr Prime Q0 = prime_of(0), Q1 = prime_of(1);
+        garner(csub(Q0, cur.r0), csub(Q1, cur.r1), cur.r2, A.C.c02m, A.C.c1m, A.C.km, x0, x1, x2);
+    } else {
+        crt_finish(A, k, cur, x0, x1, x2);
+    }
+    __syncthreads();
+    xs[0][3 + t] = x0; xs[1][3 + t] = x1; xs[2][3 + t] = x2;
+    __syncthreads();
+    // D_k and D_{k-1}
+    const u64 ad = (!FULL && A.addend && k < A.n_add) ? A.addend[k] : 0;
+    const u64 a1 = xs[1][2 + t], a2 = xs[2][1 + t];
+    // low word of D_k; its high part reaches position k+1, which
+    // recomputes it from the same LDS words (cp below)
+    const u64 dlo = x0 + a1 + a2 + ad;
+    const u64 ad1 = (!FULL && A.addend && k >= 1 && k - 1 < A.n_add) ? A.addend[k - 1] : 0;
+    const u64 p0 = xs[0][2 + t], p1 = xs[1][1 + t], p2 = xs[2][t];
+    const u64 q = p0 + p1;
+    u32 cp = q < p0 ? 1u : 0u;
+    const u64 q2 = q + p2;
+    cp += q2 < q ? 1u : 0u;
+    const u64 q3 = q2 + ad1;
+    cp += q3 < q2 ? 1u : 0u;
+    const u64 e = dlo + cp;
+    const bool g = e < dlo;
+    const bool pp = e == ~0ull;
+    const u64 G = __ballot(g), P = __ballot(pp), Am = G | P;
+    // wave carry functions
+    const u64 s0 = Am + G;
+    const u32 co0 = s0 < Am ? 1u : 0u;
+    const u64 s1 = s0 + 1;
+    const u32 co1 = co0 | (s1 == 0 ? 1u : 0u);
+    if (lane == 0) { wco[0][wv] = co0; wco[1][wv] = co1; }
+    __syncthreads();
+    // compose the four wave functions in order (a loop of constant length
+    // over wave-uniform values): the carry into this wave is the state
+    // before wave wv, the carry out of the round the state after wave 3
+    const u32 wvu = __builtin_amdgcn_readfirstlane(wv);
+    u32 c0 = cin0, c1 = cin1, n0 = cin0, n1 = cin1;
+    for (u32 w = 0; w < 4; w++) {
+        if (w == wvu) { c0 = n0; c1 = n1; }
+        const u32 f0 = wco[0][w], f1 = wco[1][w];
+        n0 = n0 ? f1 : f0;
+        n1 = n1 ? f1 : f0;
+    }
+    const u64 sw = c0 ? s1 : s0;
+    const u32 cin_lane = (u32)(((sw ^ P) >> lane) & 1u);
+    if (FULL || k < A.n_out) A.out[k] = e + cin_lane;
+    cin0 = n0; cin1 = n1;
+    __syncthreads();
+    if (t >= 253) {
+        xs[0][t - 253] = x0; xs[1][t - 253] = x1; xs[2][t - 253] = x2;
+    }
+}
+
 template <int ROUNDS = CRT_TILE_ROUNDS>
 __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn_slot, u64 (*xs)[256 + 3],
                                               u32 (*wco)[4]) {
-    const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const u32 t = threadIdx.x;
     const u64 tile_base = tile * (256 * ROUNDS);
     __syncthreads();  // LDS reuse across persistent iterations
     if (t < 3) {
@@ -69,60 +134,27 @@ __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn
     u32 cin0 = 0, cin1 = 1;  // running carry assuming tile carry-in 0 / 1
     // each round's words are loaded one round ahead, so that their HBM
     // latency overlaps the previous round's reconstruction and carries
-    CrtIn next = crt_load(A, tile_base + t);
-    for (int r = 0; r < ROUNDS; r++) {
-        const u64 k = tile_base + (u64)r * 256 + t;
-        if (k >= A.n_out && tile_base + (u64)r * 256 >= A.n_out) break;  // uniform
-        const CrtIn cur = next;
-        if (r + 1 < ROUNDS) next = crt_load(A, k + 256);
-        u64 x0, x1, x2;
-        crt_finish(A, k, cur, x0, x1, x2);
-        __syncthreads();
-        xs[0][3 + t] = x0; xs[1][3 + t] = x1; xs[2][3 + t] = x2;
-        __syncthreads();
-        // D_k and D_{k-1}
-        u64 ad = (A.addend && k < A.n_add) ? A.addend[k] : 0;
-        u64 a1 = xs[1][2 + t], a2 = xs[2][1 + t];
-        // low word of D_k; its high part reaches position k+1, which
-        // recomputes it from the same LDS words (cp below)
-        u64 dlo = x0 + a1 + a2 + ad;
-        u64 ad1 = (A.addend && k >= 1 && k - 1 < A.n_add) ? A.addend[k - 1] : 0;
-        u64 p0 = xs[0][2 + t], p1 = xs[1][1 + t], p2 = xs[2][t];
-        u64 q = p0 + p1;
-        u32 cp = q < p0 ? 1u : 0u;
-        u64 q2 = q + p2;
-        cp += q2 < q ? 1u : 0u;
-        u64 q3 = q2 + ad1;
-        cp += q3 < q2 ? 1u : 0u;
-        u64 e = dlo + cp;
-        bool g = e < dlo;
-        bool pp = e == ~0ull;
-        u64 G = __ballot(g), P = __ballot(pp), Am = G | P;
-        // wave carry functions
-        u64 s0 = Am + G;
-        u32 co0 = s0 < Am ? 1u : 0u;
-        u64 s1 = s0 + 1;
-        u32 co1 = co0 | (s1 == 0 ? 1u : 0u);
-        if (lane == 0) { wco[0][wv] = co0; wco[1][wv] = co1; }
-        __syncthreads();
-        u32 c0 = cin0, c1 = cin1;
-        for (u32 w = 0; w < wv; w++) {
-            c0 = c0 ? wco[1][w] : wco[0][w];
-            c1 = c1 ? wco[1][w] : wco[0][w];
+    const u64 tile_end = tile_base + 256 * (u64)ROUNDS;
+    if (!A.columns && !A.addend && tile_end <= A.n_res && tile_end <= A.n_out) {  // uniform
+        CrtIn cur{ld_stream(&A.r[0][tile_base + t]), ld_stream(&A.r[1][tile_base + t]),
+                  ld_stream(&A.r[2][tile_base + t])};
+#pragma unroll 2
+        for (int r = 0; r < ROUNDS; r++) {
+            const u64 k = tile_base + (u64)r * 256 + t;
+            CrtIn next{};
+            if (r + 1 < ROUNDS)
+                next = CrtIn{ld_stream(&A.r[0][k + 256]), ld_stream(&A.r[1][k + 256]), ld_stream(&A.r[2][k + 256])};
+            crt_round<true>(A, k, cur, xs, wco, cin0, cin1);
+            cur = next;
         }
-        u64 sw = c0 ? s1 : s0;
-        u32 cin_lane = (u32)(((sw ^ P) >> lane) & 1u);
-        if (k < A.n_out) A.out[k] = e + cin_lane;
-        // round carry-out for the next round
-        u32 n0 = cin0, n1 = cin1;
-        for (u32 w = 0; w < 4; w++) {
-            n0 = n0 ? wco[1][w] : wco[0][w];
-            n1 = n1 ? wco[1][w] : wco[0][w];
-        }
-        cin0 = n0; cin1 = n1;
-        __syncthreads();
-        if (t >= 253) {
-            xs[0][t - 253] = x0; xs[1][t - 253] = x1; xs[2][t - 253] = x2;
+    } else {
+        CrtIn next = crt_load(A, tile_base + t);
+        for (int r = 0; r < ROUNDS; r++) {
+            const u64 k = tile_base + (u64)r * 256 + t;
+            if (k >= A.n_out && tile_base + (u64)r * 256 >= A.n_out) break;  // uniform
+            const CrtIn cur = next;
+            if (r + 1 < ROUNDS) next = crt_load(A, k + 256);
+            crt_round<false>(A, k, cur, xs, wco, cin0, cin1);
         }
     }
     if (t == 0) *fn_slot = (u8)(cin0 | (cin1 << 1));
