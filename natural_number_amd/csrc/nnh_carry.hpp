@@ -185,19 +185,16 @@ __device__ __forceinline__ void carry_scan_body(const u8* tile_fn, u64 n_tiles, 
     const u32 co0 = s0 < Am ? 1u : 0u, co1 = co0 | (s1 == 0 ? 1u : 0u);
     if (lane == 0) chunk_fn[wv] = co0 | (co1 << 1);
     __syncthreads();
-    u32 cw = cin0;  // carry into this wave
-    for (u32 w = 0; w < wv; w++) {
+    // the 16 wave functions composed over wave-uniform values (constant trip
+    // count): cw = the carry into this wave, cc = out of the last wave
+    const u32 wvu = __builtin_amdgcn_readfirstlane(wv);
+    u32 cw = cin0, cc = cin0;
+    for (u32 w = 0; w < 16; w++) {
+        if (w == wvu) cw = cc;
         const u32 f = chunk_fn[w];
-        cw = cw ? (f >> 1) & 1u : f & 1u;
+        cc = cc ? (f >> 1) & 1u : f & 1u;
     }
-    if (cout && t == 0) {
-        u32 c = cin0;
-        for (u32 w = 0; w < 16; w++) {
-            const u32 f = chunk_fn[w];
-            c = c ? (f >> 1) & 1u : f & 1u;
-        }
-        *cout = c;
-    }
+    if (cout && t == 0) *cout = cc;
     u32 c = (u32)((((cw ? s1 : s0) ^ P) >> lane) & 1u);
     for (u64 i = b0; i < b1; i++) {
         tile_cin[i] = (u8)c;

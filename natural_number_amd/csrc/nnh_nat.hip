@@ -56,16 +56,18 @@ __global__ __launch_bounds__(256) void k_add_tiles(const AddJob* __restrict__ jo
             __syncthreads();  // previous round's readers of wco are done
             if (lane == 0) { wco[0][wv] = co0; wco[1][wv] = co1; }
             __syncthreads();
-            u32 c0 = cin0;
-            for (u32 w = 0; w < wv; w++) c0 = c0 ? wco[1][w] : wco[0][w];
-            u64 sw = c0 ? s1 : s0;
-            u32 cl = (u32)(((sw ^ P) >> lane) & 1u);
-            if (k < J.n) J.out[k] = e + cl;
-            u32 n0 = cin0, n1 = cin1;
+            // wave functions composed in a loop of constant length over
+            // wave-uniform values; c0 = the carry into this wave
+            const u32 wvu = __builtin_amdgcn_readfirstlane(wv);
+            u32 c0 = cin0, n0 = cin0, n1 = cin1;
             for (u32 w = 0; w < 4; w++) {
+                if (w == wvu) c0 = n0;
                 n0 = n0 ? wco[1][w] : wco[0][w];
                 n1 = n1 ? wco[1][w] : wco[0][w];
             }
+            u64 sw = c0 ? s1 : s0;
+            u32 cl = (u32)(((sw ^ P) >> lane) & 1u);
+            if (k < J.n) J.out[k] = e + cl;
             cin0 = n0;
             cin1 = n1;
         }
@@ -254,12 +256,17 @@ __device__ u32 wg_addsub(u64* x, u64 n, const u64* y, u64 ny, u32 sub, u32 cin, 
         __syncthreads();
         if (lane == 0) { wco[0][wv] = co0; wco[1][wv] = co1; }
         __syncthreads();
+        // wave functions composed over wave-uniform values (constant trip
+        // count): cw = the carry into this wave, c = out of the round
+        const u32 wvu = __builtin_amdgcn_readfirstlane(wv);
         u32 cw = c;
-        for (u32 w = 0; w < wv; w++) cw = cw ? wco[1][w] : wco[0][w];
+        for (u32 w = 0; w < 4; w++) {
+            if (w == wvu) cw = c;
+            c = c ? wco[1][w] : wco[0][w];
+        }
         u64 sw = cw ? s1 : s0;
         u32 cl = (u32)(((sw ^ P) >> lane) & 1u);
         if (k < n) x[k] = e + cl;
-        for (u32 w = 0; w < 4; w++) c = c ? wco[1][w] : wco[0][w];
     }
     __syncthreads();
     return c;
