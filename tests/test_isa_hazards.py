@@ -1,4 +1,5 @@
-"""SGPR wait states of the hand-written carry chains, checked in the gfx950 code.
+"""SGPR wait states of the hand-written carry chains, and scratch use, checked
+in the gfx950 code.
 
 The Montgomery products in natural_number_amd/csrc/nnh_arith.hpp are inline
 assembly that carries 64-bit column sums through SGPR carry masks
@@ -8,15 +9,17 @@ by another VALU instruction needs two wait states between them; the compiler
 inserts them (s_nop) for the code it generates, but not inside inline
 assembly, whose blocks the source orders and pads by hand.
 
-This test compiles every translation unit that contains those blocks to
-gfx950 assembly (device code only, the library's flags) and scans each
+These tests compile every HIP translation unit of the library to gfx950
+assembly (device code only, the library's flags) and scan each
 straight-line run of instructions: every VALU read of an SGPR (or VCC) that a
 VALU instruction wrote must come at least two wait states after the write
 (each instruction counts one, s_nop N counts N + 1).  It fails if a compiler
 or scheduling change ever places the blocks so that a carry is read too early.
 A sensitivity check asks the same scan for three wait states and expects it
 to find the carry reads of the inline blocks that sit exactly at two, so that
-a scanner which parsed nothing cannot pass.  No GPU is needed.
+a scanner which parsed nothing cannot pass.  The same assembly's kernel
+metadata must show no VGPR or SGPR spills and no private (scratch) segment
+for any kernel.  No GPU is needed.
 """
 import os
 import re
@@ -27,7 +30,9 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "natural_number_amd", "csrc")
-UNITS = ["nnh_mid_lazy.hip", "nnh_mid.hip", "nnh_top_lazy.hip", "nnh_top.hip", "nnh_crt.hip", "nnh_peak.hip"]
+# units whose kernels run the inline carry chains, and every other HIP unit
+ASM_UNITS = ["nnh_mid_lazy.hip", "nnh_mid.hip", "nnh_top_lazy.hip", "nnh_top.hip", "nnh_crt.hip", "nnh_peak.hip"]
+UNITS = ASM_UNITS + ["nnh_nat.hip", "nnh_mlp.hip"]
 FLAGS = ["-O3", "-std=c++20", "--offload-arch=gfx950", "--cuda-device-only", "-S",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -132,7 +137,7 @@ def gfx950_asm(tmp_path_factory):
 def test_inline_carry_chains_keep_two_wait_states(gfx950_asm):
     bad = {}
     for unit, text in gfx950_asm.items():
-        assert ";;#ASMSTART" in text, f"{unit}: no inline assembly found"
+        assert unit not in ASM_UNITS or ";;#ASMSTART" in text, f"{unit}: no inline assembly found"
         viol, _ = scan(text, need=2)
         if viol:
             bad[unit] = viol[:10]
@@ -161,3 +166,48 @@ def test_scan_flags_a_planted_early_read():
     assert len(viol) == 1 and viol[0][2] == 0
     padded = text.replace("v_addc", "s_nop 1\nv_addc")
     assert scan(padded, need=2)[0] == []
+
+
+def spills(text):
+    """(kernel, VGPR spills, SGPR spills, private segment bytes) for every
+    kernel of a gfx950 assembly file whose code uses scratch memory."""
+    meta = text[text.index("amdhsa.kernels:"):]
+    out = []
+    for m in re.finditer(r"\.name:\s+(\S+)\n(.*?)\.vgpr_spill_count:\s+(\d+)", meta, re.S):
+        body = m.group(2)
+        priv = re.search(r"\.private_segment_fixed_size:\s+(\d+)", body)
+        ssp = re.search(r"\.sgpr_spill_count:\s+(\d+)", body)
+        v, p_, s_ = int(m.group(3)), int(priv.group(1)) if priv else 0, int(ssp.group(1)) if ssp else 0
+        if v or p_ or s_:
+            out.append((m.group(1), v, s_, p_))
+    return out
+
+
+def test_no_kernel_uses_scratch(gfx950_asm):
+    # every kernel keeps its working set in registers and LDS: a spill to
+    # scratch memory (the occupancy-capped limb-loading pass once spilled two
+    # VGPRs) shows up here before it costs a GPU run
+    kernels = 0
+    bad = {}
+    for unit, text in gfx950_asm.items():
+        kernels += text.count(".vgpr_spill_count:")
+        sp = spills(text)
+        if sp:
+            bad[unit] = sp[:10]
+    assert kernels > 100, kernels
+    assert not bad, f"kernels using scratch memory: {bad}"
+
+
+def test_spill_scan_reads_the_metadata():
+    meta = "\n".join([
+        "amdhsa.kernels:",
+        "  - .name:           k_a",
+        "    .private_segment_fixed_size: 0",
+        "    .sgpr_spill_count: 0",
+        "    .vgpr_spill_count: 0",
+        "  - .name:           k_b",
+        "    .private_segment_fixed_size: 12",
+        "    .sgpr_spill_count: 0",
+        "    .vgpr_spill_count: 2",
+    ])
+    assert spills(meta) == [("k_b", 2, 0, 12)]
