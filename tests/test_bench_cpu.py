@@ -1,7 +1,7 @@
-"""bench.py's CPU baseline for the batched mix: the oracle's products over
-the first requests of the mix, compared with reference outputs (here Python
-integer products standing in for the GPU's); a changed word is counted.
-Runs without a GPU."""
+"""bench.py on the CPU: the batched mix's CPU baseline (the oracle's products
+over the first requests of the mix, compared with reference outputs, here
+Python integer products standing in for the GPU's; a changed word is
+counted), and the refusal of more than one GPU.  Runs without a GPU."""
 import os
 import sys
 
@@ -26,3 +26,13 @@ def test_batch_cpu_baseline_counts_identical_products():
     assert r["one_thread_products_per_s"] > 0 and r["all_cores_threads"] >= 1
     ref[outs[0].size + outs[1].size + 5] ^= 1 << 40   # one word of the third product
     assert bench.batch_cpu_baseline(ops, sizes, ref, max_threads=4)["gpu_products_identical"] == len(ops) - 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one GPU per box: --gpus N > 1 exits with a message, before any GPU work
+    import subprocess
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
