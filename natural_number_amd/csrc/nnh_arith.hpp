@@ -30,12 +30,27 @@ __host__ __device__ __forceinline__ u64 mk64(u32 lo, u32 hi) { return ((u64)hi <
 // Twiddles and limbs keep ordinary loads.
 // (Plain accesses where the header is compiled by a host compiler without
 // the builtins, as the CPU check of this arithmetic does.)
+// Every buffer these accesses reach is global memory (hipMalloc), and the
+// accesses say so with an address-space cast: a pointer the kernels read
+// from a job table is generic to the compiler, which would otherwise use
+// flat instructions, and those count against the LDS wait counter as well
+// as the vector-memory one, so each LDS transpose of a tile would also wait
+// for the tile's outstanding loads and stores.
 #if defined(__clang__)
-__device__ __forceinline__ u64 ld_stream(const u64* p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ void st_stream(u64* p, u64 v) { __builtin_nontemporal_store(v, p); }
+typedef __attribute__((address_space(1))) u64 gu64;
+__device__ __forceinline__ u64 ld_stream(const u64* p) { return __builtin_nontemporal_load((const gu64*)p); }
+__device__ __forceinline__ void st_stream(u64* p, u64 v) { __builtin_nontemporal_store(v, (gu64*)p); }
+template <class T>
+__device__ __forceinline__ T ld_g(const T* p) { return *(const __attribute__((address_space(1))) T*)p; }
+template <class T>
+__device__ __forceinline__ void st_g(T* p, T v) { *(__attribute__((address_space(1))) T*)p = v; }
 #else
 __device__ __forceinline__ u64 ld_stream(const u64* p) { return *p; }
 __device__ __forceinline__ void st_stream(u64* p, u64 v) { *p = v; }
+template <class T>
+__device__ __forceinline__ T ld_g(const T* p) { return *p; }
+template <class T>
+__device__ __forceinline__ void st_g(T* p, T v) { *p = v; }
 #endif
 
 // Full 64x64 -> 128-bit product from four 32x32+64 multiply-adds.

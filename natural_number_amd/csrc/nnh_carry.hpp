@@ -34,13 +34,21 @@ struct CrtArgs {
 constexpr int CRT_TILE_ROUNDS = 16;  // 16 x 256 positions per tile
 
 // The three words at position k as loaded (residues, or column words);
-// zero past n_res.
+// zero past n_res.  GMEM: the residues, addend and output are global memory
+// (the CRT kernels; accesses say so, see ld_stream); otherwise they may be
+// LDS (work-group products of the natural layer) and take generic accesses.
 struct CrtIn {
     u64 r0, r1, r2;
 };
+template <bool GMEM>
+__device__ __forceinline__ u64 crt_ld(const u64* p) {
+    if constexpr (GMEM) return ld_stream(p);
+    else return *p;
+}
+template <bool GMEM>
 __device__ __forceinline__ CrtIn crt_load(const CrtArgs& A, u64 k) {
     if (k >= A.n_res) return CrtIn{0, 0, 0};
-    return CrtIn{ld_stream(&A.r[0][k]), ld_stream(&A.r[1][k]), ld_stream(&A.r[2][k])};
+    return CrtIn{crt_ld<GMEM>(&A.r[0][k]), crt_ld<GMEM>(&A.r[1][k]), crt_ld<GMEM>(&A.r[2][k])};
 }
 __device__ __forceinline__ void crt_finish(const CrtArgs& A, u64 k, const CrtIn& in, u64& x0, u64& x1, u64& x2) {
     if (k >= A.n_res) x0 = x1 = x2 = 0;
@@ -51,15 +59,16 @@ __device__ __forceinline__ void crt_finish(const CrtArgs& A, u64 k, const CrtIn&
         garner(csub(Q0, in.r0), csub(Q1, in.r1), in.r2, A.C.c02m, A.C.c1m, A.C.km, x0, x1, x2);
     }
 }
+template <bool GMEM>
 __device__ __forceinline__ void crt_x(const CrtArgs& A, u64 k, u64& x0, u64& x1, u64& x2) {
-    crt_finish(A, k, crt_load(A, k), x0, x1, x2);
+    crt_finish(A, k, crt_load<GMEM>(A, k), x0, x1, x2);
 }
 
 // One round of a tile: positions k = tile_base + 256 r + t.  FULL: every
 // position of the tile carries residues and an output word and there is no
 // addend, so the round needs no range checks (the common case of a product's
 // tiles; the last tile and column or addend jobs take the general form).
-template <bool FULL>
+template <bool FULL, bool GMEM>
 __device__ __forceinline__ void crt_round(const CrtArgs& A, u64 k, const CrtIn& cur, u64 (*xs)[256 + 3],
                                           u32 (*wco)[4], u32& cin0, u32& cin1) {
     const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -75,12 +84,13 @@ __device__ __forceinline__ void crt_round(const CrtArgs& A, u64 k, const CrtIn& 
     xs[0][3 + t] = x0; xs[1][3 + t] = x1; xs[2][3 + t] = x2;
     __syncthreads();
     // D_k and D_{k-1}
-    const u64 ad = (!FULL && A.addend && k < A.n_add) ? A.addend[k] : 0;
+    const u64 ad = (!FULL && A.addend && k < A.n_add) ? (GMEM ? ld_g(&A.addend[k]) : A.addend[k]) : 0;
     const u64 a1 = xs[1][2 + t], a2 = xs[2][1 + t];
     // low word of D_k; its high part reaches position k+1, which
     // recomputes it from the same LDS words (cp below)
     const u64 dlo = x0 + a1 + a2 + ad;
-    const u64 ad1 = (!FULL && A.addend && k >= 1 && k - 1 < A.n_add) ? A.addend[k - 1] : 0;
+    const u64 ad1 =
+        (!FULL && A.addend && k >= 1 && k - 1 < A.n_add) ? (GMEM ? ld_g(&A.addend[k - 1]) : A.addend[k - 1]) : 0;
     const u64 p0 = xs[0][2 + t], p1 = xs[1][1 + t], p2 = xs[2][t];
     const u64 q = p0 + p1;
     u32 cp = q < p0 ? 1u : 0u;
@@ -112,7 +122,10 @@ __device__ __forceinline__ void crt_round(const CrtArgs& A, u64 k, const CrtIn& 
     }
     const u64 sw = c0 ? s1 : s0;
     const u32 cin_lane = (u32)(((sw ^ P) >> lane) & 1u);
-    if (FULL || k < A.n_out) A.out[k] = e + cin_lane;
+    if (FULL || k < A.n_out) {
+        if constexpr (GMEM) st_g(&A.out[k], e + cin_lane);
+        else A.out[k] = e + cin_lane;
+    }
     cin0 = n0; cin1 = n1;
     __syncthreads();
     if (t >= 253) {
@@ -120,7 +133,7 @@ __device__ __forceinline__ void crt_round(const CrtArgs& A, u64 k, const CrtIn& 
     }
 }
 
-template <int ROUNDS = CRT_TILE_ROUNDS>
+template <bool GMEM, int ROUNDS = CRT_TILE_ROUNDS>
 __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn_slot, u64 (*xs)[256 + 3],
                                               u32 (*wco)[4]) {
     const u32 t = threadIdx.x;
@@ -128,7 +141,7 @@ __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn
     __syncthreads();  // LDS reuse across persistent iterations
     if (t < 3) {
         u64 x0 = 0, x1 = 0, x2 = 0;
-        if (tile_base >= 3 - t) crt_x(A, tile_base - 3 + t, x0, x1, x2);
+        if (tile_base >= 3 - t) crt_x<GMEM>(A, tile_base - 3 + t, x0, x1, x2);
         xs[0][t] = x0; xs[1][t] = x1; xs[2][t] = x2;
     }
     u32 cin0 = 0, cin1 = 1;  // running carry assuming tile carry-in 0 / 1
@@ -136,25 +149,26 @@ __device__ __forceinline__ void crt_tile_body(const CrtArgs& A, u64 tile, u8* fn
     // latency overlaps the previous round's reconstruction and carries
     const u64 tile_end = tile_base + 256 * (u64)ROUNDS;
     if (!A.columns && !A.addend && tile_end <= A.n_res && tile_end <= A.n_out) {  // uniform
-        CrtIn cur{ld_stream(&A.r[0][tile_base + t]), ld_stream(&A.r[1][tile_base + t]),
-                  ld_stream(&A.r[2][tile_base + t])};
+        CrtIn cur{crt_ld<GMEM>(&A.r[0][tile_base + t]), crt_ld<GMEM>(&A.r[1][tile_base + t]),
+                  crt_ld<GMEM>(&A.r[2][tile_base + t])};
 #pragma unroll 2
         for (int r = 0; r < ROUNDS; r++) {
             const u64 k = tile_base + (u64)r * 256 + t;
             CrtIn next{};
             if (r + 1 < ROUNDS)
-                next = CrtIn{ld_stream(&A.r[0][k + 256]), ld_stream(&A.r[1][k + 256]), ld_stream(&A.r[2][k + 256])};
-            crt_round<true>(A, k, cur, xs, wco, cin0, cin1);
+                next = CrtIn{crt_ld<GMEM>(&A.r[0][k + 256]), crt_ld<GMEM>(&A.r[1][k + 256]),
+                             crt_ld<GMEM>(&A.r[2][k + 256])};
+            crt_round<true, GMEM>(A, k, cur, xs, wco, cin0, cin1);
             cur = next;
         }
     } else {
-        CrtIn next = crt_load(A, tile_base + t);
+        CrtIn next = crt_load<GMEM>(A, tile_base + t);
         for (int r = 0; r < ROUNDS; r++) {
             const u64 k = tile_base + (u64)r * 256 + t;
             if (k >= A.n_out && tile_base + (u64)r * 256 >= A.n_out) break;  // uniform
             const CrtIn cur = next;
-            if (r + 1 < ROUNDS) next = crt_load(A, k + 256);
-            crt_round<false>(A, k, cur, xs, wco, cin0, cin1);
+            if (r + 1 < ROUNDS) next = crt_load<GMEM>(A, k + 256);
+            crt_round<false, GMEM>(A, k, cur, xs, wco, cin0, cin1);
         }
     }
     if (t == 0) *fn_slot = (u8)(cin0 | (cin1 << 1));
@@ -194,7 +208,7 @@ __device__ __forceinline__ void carry_scan_body(const u8* tile_fn, u64 n_tiles, 
         const u32 f = chunk_fn[w];
         cc = cc ? (f >> 1) & 1u : f & 1u;
     }
-    if (cout && t == 0) *cout = cc;
+    if (cout && t == 0) st_g(cout, cc);
     u32 c = (u32)((((cw ? s1 : s0) ^ P) >> lane) & 1u);
     for (u64 i = b0; i < b1; i++) {
         tile_cin[i] = (u8)c;
@@ -213,15 +227,15 @@ __device__ __forceinline__ void carry_fix_body(u64* out, u64 n_out, u64 tile, u3
         u64 base = tile_base + (u64)r * 256;
         if (base >= n_out) return;
         u64 k = base + t;
-        u64 w = k < n_out ? out[k] : 0;
+        u64 w = k < n_out ? ld_g(&out[k]) : 0;
         if (t == 0) first_nz = 256;
         __syncthreads();
         if (k < n_out && w != ~0ull) atomicMin(&first_nz, t);
         __syncthreads();
         u32 f = first_nz;
         if (k < n_out) {
-            if (t < f) out[k] = 0;
-            else if (t == f) out[k] = w + 1;
+            if (t < f) st_g(&out[k], (u64)0);
+            else if (t == f) st_g(&out[k], w + 1);
         }
         if (f < 256) return;
         __syncthreads();

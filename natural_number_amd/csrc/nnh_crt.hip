@@ -26,7 +26,7 @@ template <int ROUNDS>
 __global__ __launch_bounds__(256) void k_crt_tile(CrtArgs A) {
     __shared__ u64 xs[3][256 + 3];
     __shared__ u32 wco[2][4];
-    crt_tile_body<ROUNDS>(A, blockIdx.x, A.tile_fn + blockIdx.x, xs, wco);
+    crt_tile_body<true, ROUNDS>(A, blockIdx.x, A.tile_fn + blockIdx.x, xs, wco);
 }
 
 __global__ __launch_bounds__(1024) void k_carry_scan(u8* tile_fn, u64 n_tiles, u8* tile_cin) {
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_crt_batch(const BJob* __restrict__ jobs
         A.n_add = J.n_add;
         A.tile_fn = nullptr;
         A.C = C;
-        crt_tile_body(A, it - J.tile0, tile_fn + it, xs, wco);
+        crt_tile_body<true>(A, it - J.tile0, tile_fn + it, xs, wco);
     }
 }
 
@@ -114,16 +114,16 @@ __global__ __launch_bounds__(256) void k_school_batch(const BJob* __restrict__ j
         u64 s0 = 0, s1 = 0, s2 = 0;
         for (u64 i = i0; i <= i1; i++) {
             u64 lo64, hi64;
-            mul_wide(a[i], b[c - i], lo64, hi64);
+            mul_wide(ld_g(&a[i]), ld_g(&b[c - i]), lo64, hi64);
             s0 += lo64;
             u64 h = hi64 + (s0 < lo64 ? 1u : 0u);
             s1 += h;
             s2 += s1 < h ? 1u : 0u;
         }
         const u64 n = J.n_res;
-        J.fa[c] = s0;
-        J.fa[n + c] = s1;
-        J.fa[2 * n + c] = s2;
+        st_g(&J.fa[c], s0);
+        st_g(&J.fa[n + c], s1);
+        st_g(&J.fa[2 * n + c], s2);
     }
 }
 

@@ -40,8 +40,8 @@ __global__ __launch_bounds__(256) void k_add_tiles(const AddJob* __restrict__ jo
             const u64 base = tile_base + (u64)r * 256;
             if (base >= J.n) break;  // uniform
             const u64 k = base + t;
-            u64 x = k < J.na ? J.a[k] : 0;
-            u64 y = k < J.nb ? J.b[k] : 0;
+            u64 x = k < J.na ? ld_g(&J.a[k]) : 0;
+            u64 y = k < J.nb ? ld_g(&J.b[k]) : 0;
             if (J.sub) y = ~y;
             u64 e = x + y;
             // positions past n pass the carry through, so the tile's carry
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_add_tiles(const AddJob* __restrict__ jo
             }
             u64 sw = c0 ? s1 : s0;
             u32 cl = (u32)(((sw ^ P) >> lane) & 1u);
-            if (k < J.n) J.out[k] = e + cl;
+            if (k < J.n) st_g(&J.out[k], e + cl);
             cin0 = n0;
             cin1 = n1;
         }
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(1024) void k_add_scan(const AddJob* __restrict__ jo
     for (u32 j = blockIdx.x; j < njobs; j += gridDim.x) {
         const AddJob J = jobs ? jobs[j] : one;
         const u64 t0 = J.tile0, t1 = j + 1 < njobs ? jobs[j + 1].tile0 : ntiles;
-        const u32 cin = J.sub ? 1u : (J.cin_ptr ? *J.cin_ptr : J.cin);
+        const u32 cin = J.sub ? 1u : (J.cin_ptr ? ld_g(J.cin_ptr) : J.cin);
         __syncthreads();
         carry_scan_body(tile_fn + t0, t1 - t0, tile_cin + t0, chunk_fn, cin, J.cout);
     }
@@ -334,7 +334,7 @@ __device__ void wg_mul(u64* out, u64 n_out, const u64* a, u64 na, const u64* b, 
     A.n_add = n_add;
     A.tile_fn = nullptr;
     A.C = CrtConsts{};
-    crt_tile_body(A, 0, fn, xs, wco);
+    crt_tile_body<false>(A, 0, fn, xs, wco);  // columns, addend and output may be LDS
     __syncthreads();
 }
 

@@ -168,7 +168,7 @@ __device__ __forceinline__ void load_limbs(const Prime& P, u64 (&v)[16], const u
     CFor<0, N>::run([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         const u64 x = xs(j);
-        w[j] = a[x < an ? x : an - 1];
+        w[j] = ld_g(&a[x < an ? x : an - 1]);
     });
     CFor<0, N>::run([&](auto jc) {
         constexpr int j = decltype(jc)::value;
