@@ -19,7 +19,8 @@ A sensitivity check asks the same scan for three wait states and expects it
 to find the carry reads of the inline blocks that sit exactly at two, so that
 a scanner which parsed nothing cannot pass.  The same assembly's kernel
 metadata must show no VGPR or SGPR spills and no private (scratch) segment
-for any kernel.  No GPU is needed.
+for any kernel, and no transform or CRT kernel may access memory with flat
+instructions.  No GPU is needed.
 """
 import os
 import re
@@ -211,3 +212,20 @@ def test_spill_scan_reads_the_metadata():
         "    .vgpr_spill_count: 2",
     ])
     assert spills(meta) == [("k_b", 2, 0, 12)]
+
+
+def test_transform_and_crt_kernels_use_global_memory_instructions(gfx950_asm):
+    # Pointers read from a job table are generic to the compiler; without the
+    # address-space casts of the residue, limb and output accesses the
+    # batched kernels fall back to flat instructions, which also count
+    # against the LDS wait counter (each LDS transpose then waits for the
+    # tile's memory operations).  No transform or CRT kernel may use them.
+    bad = {}
+    for unit in ("nnh_mid_lazy.hip", "nnh_mid.hip", "nnh_top_lazy.hip", "nnh_top.hip", "nnh_crt.hip"):
+        text = gfx950_asm[unit]
+        for m in re.finditer(r"^(_ZN\w+):", text, re.M):
+            end = text.find(".Lfunc_end", m.end())
+            n = len(re.findall(r"^\s+flat_(load|store|atomic)", text[m.end():end], re.M))
+            if n:
+                bad.setdefault(unit, []).append((m.group(1)[:60], n))
+    assert not bad, f"kernels with flat memory instructions: {bad}"
