@@ -200,6 +200,15 @@ struct Item {
     bool piece;
 };
 
+// Every buffer carved from the batch workspace starts on a 64-word (512-byte)
+// boundary.  The transform kernels move rows of 16-32 consecutive words
+// (128-256 bytes): packed after the odd-sized schoolbook and chunk buffers,
+// a bucket's residues could start mid cache line, and every row segment then
+// touched one line more (the 2^19-word bucket's strided passes ran ~2x
+// slower per job than the same bucket planned alone).
+constexpr u64 WS_ALIGN_WORDS = 64;
+u64 ws_align(u64 w) { return (w + WS_ALIGN_WORDS - 1) & ~(WS_ALIGN_WORDS - 1); }
+
 u64* wsptr(u64 off) { return reinterpret_cast<u64*>(static_cast<uintptr_t>(off)); }
 u64 wsoff(const void* p) { return static_cast<u64>(reinterpret_cast<uintptr_t>(p)); }
 
@@ -252,7 +261,7 @@ int plan_batch(const IReq* reqs, u64 n, const PolicyTable* pol, u64 count, Batch
                 R.piece = (1ull << sp) - small + 1;
                 R.pieces = pieces;
                 R.tmp_off = ws;
-                ws += r.an + r.bn - R.piece;
+                ws = ws_align(ws + r.an + r.bn - R.piece);
             } else {
                 R.s = std::max(ntt_scale(r.an, r.bn), MIN_MID_BITS);
             }
@@ -319,7 +328,7 @@ int plan_batch(const IReq* reqs, u64 n, const PolicyTable* pol, u64 count, Batch
             J.col0 = cols;
             cols += J.n_res;
             J.fa = wsptr(ws);
-            ws += 3 * J.n_res;
+            ws = ws_align(ws + 3 * J.n_res);
         } else {
             const u64 L = 1ull << it.s;
             J.sqr = sq;

@@ -123,6 +123,15 @@ static void check_plan(const std::vector<IReq>& R, const BatchPlan& P, const cha
             iv.push_back({off(J.fb), off(J.fb) + 3 * Lj});
         }
     }
+    // every residue and transform buffer starts on a 64-word boundary (the
+    // transform kernels' row segments then never straddle an extra line),
+    // and so does every chunked request's odd-piece buffer
+    for (uint32_t j = 0; j < P.jobs.size(); j++) {
+        const BJob& J = P.jobs[j];
+        REQUIRE(off(J.fa) % 64 == 0, w + ": residues aligned");
+        if (!J.school) REQUIRE(off(J.fb) % 64 == 0, w + ": transform aligned");
+    }
+    for (const AddJob& A : P.adds) REQUIRE(off(A.b) % 64 == 0, w + ": odd-piece buffer aligned");
     // odd-piece buffers
     for (uint32_t j = 0; j < P.n_req_jobs; j++)
         if (P.out_in_ws[j]) REQUIRE(off(P.jobs[j].out) + P.jobs[j].n_out <= P.ws_words, w + ": odd window in ws");
