@@ -77,7 +77,7 @@ int nnh_sqr(uint64_t* dst, const uint64_t* a, uint64_t an);
 #define NNH_ALGO_NTT 3         /* one transform for the whole product */
 #define NNH_ALGO_NTT_SPLIT 4   /* the reference's one-level unbalanced split (_mul_NTT) */
 #define NNH_ALGO_NTT_CHUNK 5   /* larger operand in pieces, all pieces one batch sharing the shorter
-                                  operand's transform; synchronises the stream */
+                                  operand's transform (planned on the host, queued without waiting) */
 #define NNH_ALGO_REFERENCE 6   /* the reference's fixed thresholds, whatever policy is installed */
 #define NNH_ALGO_MODEL 7       /* the installed cost-model policy (NNH_ERR_ARG if none); param = count of
                                   back-to-back products the choice assumes (0 = 1) */
@@ -152,9 +152,9 @@ int nnh_arm_call(int arm, uint64_t an, uint64_t bn, int* algo, int* param);
 
 /* Device-pointer variants: operands and result already in HBM, work is
  * queued on `stream` (a hipStream_t, NULL = the library's stream) and the
- * call returns without synchronising, except when the dispatch (the
- * installed cost-model policy) takes the chunked arm, which plans its
- * pieces as a batch and synchronises the stream.  d_dst must not overlap
+ * call returns without synchronising (when the dispatch, the installed
+ * cost-model policy, takes the chunked arm, the host plans its pieces as a
+ * batch before the call returns).  d_dst must not overlap
  * d_a or d_b (the split path writes its first half before loading the
  * second).  Calls on different streams are ordered by the library (each
  * waits for the previous call's work in the shared workspace). */

@@ -125,6 +125,14 @@ struct Device {
     int nwork = NWORK;
     hipStream_t work[NWORK]{};
     hipEvent_t ev_bfork = nullptr, ev_bjoin[NWORK]{};
+    // Internal batches (dev_batch) upload their job tables from this pinned
+    // staging buffer on the caller's stream and return without waiting;
+    // ev_tab marks the last upload, which must finish before the staging
+    // buffer is rewritten.
+    u8* tab_host = nullptr;
+    u64 tab_bytes = 0;
+    hipEvent_t ev_tab = nullptr;
+    bool tab_pending = false;
     bool two_streams = true;
     RootBases rb{};
     CrtConsts crt{};
@@ -358,6 +366,7 @@ static int device_init(int dev) {
         if (!g_dev.ev_bjoin[w]) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_bjoin[w], hipEventDisableTiming));
     }
     if (!g_dev.ev_bfork) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_bfork, hipEventDisableTiming));
+    if (!g_dev.ev_tab) HIPCHK(hipEventCreateWithFlags(&g_dev.ev_tab, hipEventDisableTiming));
     {
         const char* e = std::getenv("NNH_ONE_STREAM");
         g_dev.two_streams = !(e && e[0] == '1');
@@ -997,7 +1006,8 @@ static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
 
 // flags bit 0: capture into a hipGraph; bit 1: internal batch (buffers from
 // the device's reusable batch workspace; no policy); bit 2: plan each
-// request's arm with the installed policy.  forced_arm >= 0: every request
+// request's arm with the installed policy; bit 4: leave the job tables to
+// the caller (upload_tables, on its stream).  forced_arm >= 0: every request
 // runs that arm (timing).
 static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch** out, int forced_arm = -1) {
     nnh_batch* B = new nnh_batch();
@@ -1053,10 +1063,12 @@ static int batch_create(Device& D, const IReq* reqs, u64 n, int flags, nnh_batch
     rebase_plan(P, B->ws);
     std::vector<u32> sj(P.n_school);
     for (u32 i = 0; i < P.n_school; i++) sj[i] = i;  // school jobs sort first
-    HIPCHK(hipMemcpy(B->d_jobs, P.jobs.data(), P.jobs.size() * sizeof(BJob), hipMemcpyHostToDevice));
-    if (P.n_school) HIPCHK(hipMemcpy(B->d_sj, sj.data(), sj.size() * 4, hipMemcpyHostToDevice));
-    if (!P.adds.empty())
-        HIPCHK(hipMemcpy(B->d_adds, P.adds.data(), P.adds.size() * sizeof(AddJob), hipMemcpyHostToDevice));
+    if (!(flags & 16)) {
+        HIPCHK(hipMemcpy(B->d_jobs, P.jobs.data(), P.jobs.size() * sizeof(BJob), hipMemcpyHostToDevice));
+        if (P.n_school) HIPCHK(hipMemcpy(B->d_sj, sj.data(), sj.size() * 4, hipMemcpyHostToDevice));
+        if (!P.adds.empty())
+            HIPCHK(hipMemcpy(B->d_adds, P.adds.data(), P.adds.size() * sizeof(AddJob), hipMemcpyHostToDevice));
+    }
     if (flags & 1) {
         HIPCHK(hipStreamCreateWithFlags(&B->cap_stream, hipStreamNonBlocking));
         HIPCHK(hipStreamBeginCapture(B->cap_stream, hipStreamCaptureModeThreadLocal));
@@ -1266,21 +1278,57 @@ int dev_mul_scale(u64* out, const u64* a, u64 an, const u64* b, u64 bn, int s, u
     return NNH_OK;
 }
 
+// Job tables of an internal batch, copied on st from the device's pinned
+// staging buffer (job table, schoolbook list, additions).  The buffers they
+// land in are shared by every internal batch: the copy is ordered on st after
+// the previous batch's work, whose last launches run on st (the lanes join
+// it), and a call on another stream first waits for the previous call's work
+// (the workspace hand-over of the entry points).
+static int upload_tables(Device& D, nnh_batch* B, hipStream_t st) {
+    const BatchPlan& P = B->plan;
+    const u64 jb = P.jobs.size() * sizeof(BJob), sb = (u64)P.n_school * 4, ab = P.adds.size() * sizeof(AddJob);
+    const u64 o_sj = (jb + 255) & ~255ull, o_ad = (o_sj + sb + 255) & ~255ull, total = o_ad + ab;
+    if (D.tab_pending) {
+        HIPCHK(hipEventSynchronize(D.ev_tab));
+        D.tab_pending = false;
+    }
+    if (total > D.tab_bytes) {
+        if (D.tab_host) (void)hipHostFree(D.tab_host);
+        D.tab_host = nullptr;
+        D.tab_bytes = 0;
+        const u64 cap = std::bit_ceil(std::max<u64>(total, 65536));
+        HIPCHK(hipHostMalloc((void**)&D.tab_host, cap, hipHostMallocDefault));
+        D.tab_bytes = cap;
+    }
+    std::memcpy(D.tab_host, P.jobs.data(), jb);
+    u32* sj = reinterpret_cast<u32*>(D.tab_host + o_sj);
+    for (u32 i = 0; i < P.n_school; i++) sj[i] = i;  // school jobs sort first
+    if (ab) std::memcpy(D.tab_host + o_ad, P.adds.data(), ab);
+    HIPCHK(hipMemcpyAsync(B->d_jobs, D.tab_host, jb, hipMemcpyHostToDevice, st));
+    if (sb) HIPCHK(hipMemcpyAsync(B->d_sj, D.tab_host + o_sj, sb, hipMemcpyHostToDevice, st));
+    if (ab) HIPCHK(hipMemcpyAsync(B->d_adds, D.tab_host + o_ad, ab, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(D.ev_tab, st));
+    D.tab_pending = true;
+    return NNH_OK;
+}
+
+// An internal batch, queued on st without waiting for it: planning the next
+// call overlaps this one's kernels (a synchronising version left the GPU
+// idle ~0.1 ms per call while the host planned and uploaded).
 int dev_batch(const IReq* reqs, u64 n, hipStream_t st) {
     if (!n) return NNH_OK;
     nnh_batch* B = nullptr;
-    int rc = batch_create(g_dev, reqs, n, 2, &B);
+    int rc = batch_create(g_dev, reqs, n, 2 | 16, &B);
     if (rc) return rc;
-    rc = batch_enqueue(g_dev, B, st);
-    hipError_t e = hipGetLastError();
-    if (rc) {
+    rc = upload_tables(g_dev, B, st);
+    if (!rc) rc = batch_enqueue(g_dev, B, st);
+    const hipError_t e = hipGetLastError();
+    if (rc || e != hipSuccess) {
         (void)hipStreamSynchronize(st);
         batch_free(B);
-        return rc;
+        return rc ? rc : fail(NNH_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    batch_free(B);
-    if (e != hipSuccess) return fail(NNH_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
+    batch_free(B);  // the plan's host side only: the device tables stay in the shared buffers
     return NNH_OK;
 }
 
@@ -1374,6 +1422,10 @@ int nnh_shutdown(void) {
         if (b->p) (void)hipFree(b->p);
         *b = Buf{};
     }
+    if (g_dev.tab_host) (void)hipHostFree(g_dev.tab_host);
+    g_dev.tab_host = nullptr;
+    g_dev.tab_bytes = 0;
+    g_dev.tab_pending = false;
     return NNH_OK;
 }
 
