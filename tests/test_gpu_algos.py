@@ -96,3 +96,31 @@ def test_butterfly_peak_rates(nnh):
         assert 1e10 < r < 1e14, (pr, r)
     with pytest.raises(nnh.NNHError):
         nnh.butterfly_peak(3, 32)
+
+
+def test_chunked_products_queued_back_to_back(nnh, oracle):
+    # Chunked products plan their pieces as internal batches that are queued
+    # without waiting: each one's job tables go up on the stream from one
+    # pinned staging buffer, which the next call rewrites (and grows, for
+    # the larger tables here) once the previous upload is done.  Several
+    # such products queued with no synchronisation between them must each
+    # match the oracle.
+    shapes = [(20000, 1000, 11), (400000, 300, 10), (100000, 300, 10), (200000, 1500, 11), (60000, 350, 9)]
+    rng = np.random.default_rng(77)
+    ops = [(rand_limbs(rng, an), rand_limbs(rng, bn), sp) for an, bn, sp in shapes]
+    st = nnh.Stream()
+    bufs = []
+    for a, b, sp in ops:
+        da, db, dout = nnh.DeviceWords(a.size, a), nnh.DeviceWords(b.size, b), nnh.DeviceWords(a.size + b.size)
+        bufs.append((da, db, dout))
+    nnh.synchronize()
+    for (a, b, sp), (da, db, dout) in zip(ops, bufs):
+        nnh.mul_algo_device_async(dout.ptr, da.ptr, a.size, db.ptr, b.size, nnh.ALGO_NTT_CHUNK, sp, st.ptr)
+    st.synchronize()
+    for (a, b, sp), (da, db, dout) in zip(ops, bufs):
+        say(f"queued chunked {a.size}x{b.size} sp={sp}")
+        assert np.array_equal(dout.download(), oracle.mul(a, b)), (a.size, b.size, sp)
+    for t in bufs:
+        for x in t:
+            x.free()
+    st.close()
