@@ -154,7 +154,8 @@ int nnh_arm_call(int arm, uint64_t an, uint64_t bn, int* algo, int* param);
  * queued on `stream` (a hipStream_t, NULL = the library's stream) and the
  * call returns without synchronising (when the dispatch, the installed
  * cost-model policy, takes the chunked arm, the host plans its pieces as a
- * batch before the call returns).  d_dst must not overlap
+ * batch before the call returns; that arm refuses a stream under graph
+ * capture with NNH_ERR_ARG).  d_dst must not overlap
  * d_a or d_b (the split path writes its first half before loading the
  * second).  Calls on different streams are ordered by the library (each
  * waits for the previous call's work in the shared workspace). */

@@ -815,6 +815,9 @@ static int ntt_chunked(Device& D, u64* out, const u64* a, u64 an, const u64* b, 
     }
     const u64 G = (P + bn + P - 1) / P;
     if (G > 4) return ntt_chunked_seq(D, out, a, an, b, bn, sp, st);
+    // the pieces run as an internal batch, which a graph cannot hold (refused
+    // here, before anything is enqueued)
+    if (capturing(st)) return fail(NNH_ERR_ARG, "an internal batch (chunked arm) cannot be captured into a graph");
     if ((rc = ensure(D.tmp, (G - 1) * n + 64)) || (rc = ensure(D.tiles, 2 * ((n + LIMB_TILE - 1) / LIMB_TILE) + 16)))
         return rc;
     auto buf = [&](u64 g) { return g == 0 ? out : D.tmp.p + (g - 1) * n; };
@@ -1317,6 +1320,9 @@ static int upload_tables(Device& D, nnh_batch* B, hipStream_t st) {
 // idle ~0.1 ms per call while the host planned and uploaded).
 int dev_batch(const IReq* reqs, u64 n, hipStream_t st) {
     if (!n) return NNH_OK;
+    // A captured upload would read the staging buffer at every replay,
+    // after later calls have rewritten it.
+    if (capturing(st)) return fail(NNH_ERR_ARG, "an internal batch (chunked arm) cannot be captured into a graph");
     nnh_batch* B = nullptr;
     int rc = batch_create(g_dev, reqs, n, 2 | 16, &B);
     if (rc) return rc;

@@ -124,3 +124,37 @@ def test_chunked_products_queued_back_to_back(nnh, oracle):
         for x in t:
             x.free()
     st.close()
+
+
+def test_chunked_arm_refuses_graph_capture(nnh, oracle):
+    # The chunked arm's pieces go through an internal batch whose job tables
+    # are uploaded from a staging buffer that later calls rewrite: under
+    # stream capture the call is refused before anything is enqueued, and
+    # the (empty) capture ends normally.
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    an, bn, sp = 100000, 1000, 11  # G = 2 groups of pieces: the batched form
+    rng = np.random.default_rng(5)
+    a, b = rand_limbs(rng, an), rand_limbs(rng, bn)
+    da, db, dout = nnh.DeviceWords(an, a), nnh.DeviceWords(bn, b), nnh.DeviceWords(an + bn)
+    st = nnh.Stream()
+    nnh.synchronize()
+    assert hip.hipStreamBeginCapture(ctypes.c_void_p(st.ptr), 2) == 0  # hipStreamCaptureModeRelaxed
+    rc = nnh.lib().nnh_mul_algo_device_async(dout.ptr, da.ptr, an, db.ptr, bn, nnh.ALGO_NTT_CHUNK, sp, st.ptr)
+    graph = ctypes.c_void_p()
+    assert hip.hipStreamEndCapture(ctypes.c_void_p(st.ptr), ctypes.byref(graph)) == 0
+    if graph.value:
+        hip.hipGraphDestroy(graph)
+    assert rc == 1, rc  # NNH_ERR_ARG
+    assert "captured" in nnh.lib().nnh_last_error().decode()
+    # the stream and the library still work
+    nnh.mul_algo_device_async(dout.ptr, da.ptr, an, db.ptr, bn, nnh.ALGO_NTT_CHUNK, sp, st.ptr)
+    st.synchronize()
+    got = dout.download()
+    st.close()
+    for x in (da, db, dout):
+        x.free()
+    assert np.array_equal(got, oracle.mul(a, b))
