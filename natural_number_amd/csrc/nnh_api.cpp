@@ -953,9 +953,12 @@ static void batch_free(nnh_batch* B) {
 static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
     const BatchPlan& P = B->plan;
     const bool fan = D.two_streams && D.nwork > 0 && P.buckets.size() > 0;
+    // the buckets' two chains take lanes st, work[0], work[1], ... in turn:
+    // only the workers they reach are forked and joined
+    const int used = fan ? (int)std::min<u64>((u64)D.nwork, 2 * P.buckets.size() - 1) : 0;
     if (fan) {
         HIPCHK(hipEventRecord(D.ev_bfork, st));
-        for (int w = 0; w < D.nwork; w++) HIPCHK(hipStreamWaitEvent(D.work[w], D.ev_bfork, 0));
+        for (int w = 0; w < used; w++) HIPCHK(hipStreamWaitEvent(D.work[w], D.ev_bfork, 0));
     }
     // lanes: st and the workers
     int lane = 0;
@@ -998,7 +1001,7 @@ static int batch_enqueue(Device& D, nnh_batch* B, hipStream_t st) {
         }
     }
     if (fan)
-        for (int w = 0; w < D.nwork; w++) {
+        for (int w = 0; w < used; w++) {
             HIPCHK(hipEventRecord(D.ev_bjoin[w], D.work[w]));
             HIPCHK(hipStreamWaitEvent(st, D.ev_bjoin[w], 0));
         }
