@@ -118,11 +118,13 @@ struct Device {
     // stream2; NNH_BATCH_STREAMS=0..4 sets nwork): one bucket's p0/p1 chain
     // on one lane, its p2 chain on the next, forked from and joined back into
     // the caller's stream; under graph capture they become parallel branches.
-    // 4 workers: -3 % on the 4096-product mix.  (Four freshly created worker
-    // streams instead of three plus stream2 slowed the two-stream product by
-    // 3 % with the default 4 hardware queues.)
+    // 3 workers by default, four lanes for the default 4 hardware queues:
+    // the 4096-product mix 18.55-18.59 ms against 18.67-18.81 ms with 4
+    // workers and 18.58-18.68 ms with 2 (workers against none: -3 %).  (Four
+    // freshly created worker streams instead of three plus stream2 slowed
+    // the two-stream product by 3 %.)
     static constexpr int NWORK = 4;
-    int nwork = NWORK;
+    int nwork = 3;
     hipStream_t work[NWORK]{};
     hipEvent_t ev_bfork = nullptr, ev_bjoin[NWORK]{};
     // Internal batches (dev_batch) upload their job tables from this pinned
